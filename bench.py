@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark of the three-body hot path on one MI355X.
+
+Workload (one "step" = one simulation step: predictor -> cutoff sweep -> update):
+P3BCA, cutoff 2, N = 32768 particles uniform in a 25.4^3 box (density 2.0),
+mass 1, velocities 0, dt = 0.001, g = (0, -9.81, 0); particles from the
+reference generator's algorithm (mt19937, seed 926762934) written through the
+CSV text format.  Warm-up steps run first, then the state is reset so the K
+timed steps are steps 0..K-1 of that trajectory.
+
+Printed JSON line:
+  value        unique triplets/s of the FP64 (squared-distance) path, inputs
+               resident in HBM, host loop included (the step is synchronous)
+  exact        the bit-exact FP64 path (reference expression and order)
+  fp32_mixed   the FP32-mixed path, its speedup over FP64 and its force /
+               energy error against the bit-exact path on the initial state
+  roofline     the force kernel: 115 FLOP per unique triplet / its average
+               duration from HIP events on the engine's stream, against the
+               FP64 vector peak
+  cpu_baseline the reference-shaped CPU sweep (oracle restatement of the
+               reference's one-process i<j<k loop with AoS particles and a
+               virtual per-triplet call), single thread, on a bounded sample
+               of rows of step 0, extrapolated by the exact candidate count
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "unique triplets/s + ms/step, P3BCA c=2 N=32k; FP32-mix vs FP64 at 1e-6 F err"
+FLOP_PER_TRIPLET = 115
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+N, BOX, CUTOFF, DT, SEED = 32768, 25.4, 2.0, 0.001, 926762934
+G = (0.0, -9.81, 0.0)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload():
+    from threebodymdalgorithms_amd import generators
+    return generators.csv_rounded(generators.uniform(N, BOX, SEED))
+
+
+def timed_steps(eng, P, precision, steps, warmup, sync):
+    eng.set_particles(P)
+    eng.configure("p3bca", cutoff=CUTOFF, precision=precision)
+    for _ in range(warmup):
+        eng.step(DT, G)
+    eng.set_particles(P)
+    eng.set_profiling(True)
+    sync()
+    t0 = time.perf_counter()
+    counts = [eng.step(DT, G)[1] for _ in range(steps)]
+    sync()
+    t1 = time.perf_counter()
+    prof = eng.get_profile()
+    eng.set_profiling(False)
+    secs = t1 - t0
+    total = int(sum(counts))
+    fk_ms = prof["forces_ms"] / max(prof["steps"], 1)
+    t_avg = prof["triplets"] / max(prof["steps"], 1)
+    return {
+        "triplets_per_s": total / secs,
+        "ms_per_step": 1e3 * secs / steps,
+        "triplets_per_step": total / steps,
+        "counts": counts,
+        "force_kernel_ms": fk_ms,
+        "neighbour_ms": prof["neighbours_ms"] / max(prof["steps"], 1),
+        "integrate_ms": (prof["predict_ms"] + prof["update_ms"]) / max(prof["steps"], 1),
+        "force_kernel_tflops": FLOP_PER_TRIPLET * t_avg / (fk_ms * 1e-3) / 1e12 if fk_ms > 0 else 0.0,
+    }
+
+
+def fp32_accuracy(eng, P):
+    """FP32-mixed forces and energy against the bit-exact FP64 path on P."""
+    eng.set_particles(P)
+    eng.configure("p3bca", cutoff=CUTOFF, precision="exact")
+    E64, f64, _, _ = eng.energy_grad()
+    eng.configure("p3bca", cutoff=CUTOFF, precision="fp32")
+    E32, f32, _, _ = eng.energy_grad()
+    num = np.linalg.norm(f32 - f64, axis=1)
+    den = np.linalg.norm(f64, axis=1)
+    rel = num / np.where(den > 0, den, 1.0)
+    return {"max_rel_force_err": float(rel.max()), "rms_rel_force_err": float(np.sqrt(np.mean(rel ** 2))),
+            "rel_energy_err": float(abs(E32 - E64) / abs(E64))}
+
+
+def cpu_baseline(P, count0, target_s):
+    """Reference-shaped single-thread sweep on rows 0..R-1 of step 0."""
+    so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(so):
+        return None
+    lib = ctypes.CDLL(so)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.orc_sweep_brute_rows.argtypes = [ctypes.c_int64, dp, ctypes.c_double, ctypes.c_double, dp, dp, dp, dp,
+                                         ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
+    lib.orc_sweep_brute_rows.restype = ctypes.c_uint64
+    lib.orc_domain_size.argtypes = [ctypes.c_int64, dp, dp, dp]
+    pos = np.ascontiguousarray(P, dtype=np.float64)
+    L, lo = np.zeros(3), np.zeros(3)
+    lib.orc_domain_size(N, pos.ctypes.data_as(dp), L.ctypes.data_as(dp), lo.ctypes.data_as(dp))
+    f = [np.zeros((N, 3)) for _ in range(3)]
+    acc = ctypes.c_uint64(0)
+
+    def run(rows):
+        t0 = time.perf_counter()
+        cand = lib.orc_sweep_brute_rows(N, pos.ctypes.data_as(dp), 1.0, CUTOFF, L.ctypes.data_as(dp),
+                                        f[0].ctypes.data_as(dp), f[1].ctypes.data_as(dp), f[2].ctypes.data_as(dp),
+                                        rows, ctypes.byref(acc))
+        return time.perf_counter() - t0, int(cand)
+
+    t1, _ = run(1)
+    rows = max(1, min(N - 2, int(target_s / max(t1, 1e-3))))
+    secs, cand = run(rows)
+    total_cand = N * (N - 1) * (N - 2) // 6
+    step_s = secs * total_cand / cand
+    return {"value": count0 / step_s, "unit": "unique triplets/s", "cores": 1, "kind": "port",
+            "sample": f"reference-shaped brute i<j<k sweep, rows 0..{rows - 1} of step 0 ({cand:.3e} of "
+                      f"{total_cand:.3e} candidates, {secs:.1f} s), extrapolated by candidate count to "
+                      f"{step_s:.0f} s per step",
+            "ms_per_step": step_s * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-baseline-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-variants", action="store_true", help="time the FP64 path only")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import threebodymdalgorithms_amd as tb3
+
+    def sync():
+        torch.cuda.synchronize()
+
+    P = workload()
+    log(f"bench: N={N} box={BOX} cutoff={CUTOFF} steps={args.steps} warmup={args.warmup}")
+    with tb3.Engine(0) as eng:
+        fp64 = timed_steps(eng, P, "fp64", args.steps, args.warmup, sync)
+        log(f"bench: fp64 {fp64['ms_per_step']:.3f} ms/step {fp64['triplets_per_s']:.4e} triplets/s "
+            f"force kernel {fp64['force_kernel_ms']:.3f} ms")
+        out_exact = out_fp32 = None
+        if not args.skip_variants:
+            ex = timed_steps(eng, P, "exact", args.steps, args.warmup, sync)
+            log(f"bench: exact {ex['ms_per_step']:.3f} ms/step force kernel {ex['force_kernel_ms']:.3f} ms")
+            if ex["counts"] != fp64["counts"]:
+                log(f"bench: WARNING exact and fp64 counts differ: {ex['counts']} vs {fp64['counts']}")
+            f32 = timed_steps(eng, P, "fp32", args.steps, args.warmup, sync)
+            acc = fp32_accuracy(eng, P)
+            log(f"bench: fp32 {f32['ms_per_step']:.3f} ms/step force kernel {f32['force_kernel_ms']:.3f} ms {acc}")
+            out_exact = {"value": ex["triplets_per_s"], "ms_per_step": ex["ms_per_step"],
+                         "force_kernel_ms": ex["force_kernel_ms"],
+                         "force_kernel_tflops": ex["force_kernel_tflops"]}
+            out_fp32 = {"value": f32["triplets_per_s"], "ms_per_step": f32["ms_per_step"],
+                        "force_kernel_ms": f32["force_kernel_ms"],
+                        "speedup_vs_fp64_step": fp64["ms_per_step"] / f32["ms_per_step"],
+                        "speedup_vs_fp64_force_kernel": fp64["force_kernel_ms"] / f32["force_kernel_ms"],
+                        **acc}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline ...")
+        cpu = cpu_baseline(P, fp64["counts"][0], args.cpu_baseline_seconds)
+    achieved = fp64["force_kernel_tflops"]
+    rec = {
+        "metric": METRIC,
+        "value": fp64["triplets_per_s"],
+        "unit": "unique triplets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fp64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "P3BCA cutoff 2, N=32768 uniform in 25.4^3 (density 2), dt 0.001, g (0,-9.81,0)",
+                   "algorithm": "p3bca", "n": N, "cutoff": CUTOFF, "precision": "fp64",
+                   "triplets_per_step": fp64["triplets_per_step"], "counts": fp64["counts"],
+                   "force_kernel_ms": fp64["force_kernel_ms"], "neighbour_ms": fp64["neighbour_ms"],
+                   "integrate_ms": fp64["integrate_ms"]},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_forces_fast_nbr<double>", "flop_per_triplet": FLOP_PER_TRIPLET},
+        "cpu_baseline": cpu,
+        "exact": out_exact,
+        "fp32_mixed": out_fp32,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

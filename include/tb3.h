@@ -1,0 +1,160 @@
+/*
+ * tb3.h -- C ABI of the MI355X three-body (Axilrod-Teller) force engine.
+ *
+ * One context owns one GPU's device buffers for one particle system.  All
+ * calls are made from one host thread per context, are ordered on the
+ * context's HIP stream and are synchronous on return unless stated otherwise.
+ * No exceptions cross this boundary: every call returns a tb3_status and
+ * tb3_last_error() gives the message of the last failure.
+ *
+ * Citations name the reference interface each entry point replaces
+ * (ThreeBodyMDAlgorithms, paths relative to its src/ directory).  At one
+ * process the reference runs Simulation::Start (simulation/Simulation.cpp:28-81):
+ * predictor -> Algorithm::SimulationStep -> Update, one buffer triplet per
+ * step (P3BCA.cpp:31-32, 89; AUTA.cpp:268-302; NATA.cpp:112-130).
+ */
+#ifndef TB3_H
+#define TB3_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TB3_OK = 0,
+    TB3_EINVAL = 1,       /* bad argument or call order */
+    TB3_ENOMEM = 2,       /* device or host allocation failed */
+    TB3_EHIP = 3,         /* a HIP runtime call failed */
+    TB3_EUNSUPPORTED = 4, /* valid request this build cannot serve */
+    TB3_ENODEV = 5        /* no usable GPU */
+} tb3_status;
+
+/* Same numbering as the reference's AlgorithmType (utility/enums.hpp:1). */
+typedef enum { TB3_NATA = 0, TB3_P3BCA = 1, TB3_AUTA = 2 } tb3_algo;
+
+typedef enum {
+    /* Bit-identical to the reference's one-process sweep: the reference's
+     * per-triplet expression (potential/AxilrodTeller.cpp:22-93) and its
+     * per-particle accumulation order (algorithm/Algorithm.cpp:72-139, 246-248). */
+    TB3_FP64_EXACT = 0,
+    /* FP64 throughout, squared-distance formulation, any accumulation order. */
+    TB3_FP64 = 1,
+    /* Per-triplet arithmetic in FP32 on pair differences; FP64 differences,
+     * cutoff test, accumulation and integration. */
+    TB3_FP32_MIXED = 2
+} tb3_precision;
+
+typedef struct tb3_ctx tb3_ctx;
+
+/* Library version string, e.g. "tb3 1.0.0 gfx950". */
+const char *tb3_version(void);
+
+/* Number of visible GPUs (hipGetDeviceCount). */
+tb3_status tb3_device_count(int *count);
+
+/* Replaces the construction of Simulation + topology + decomposition
+ * (main.cpp:32-105, Simulation.hpp:28-31) for one GPU.  Creates a context on
+ * `device` with its own stream. */
+tb3_status tb3_create(int device, tb3_ctx **out);
+tb3_status tb3_destroy(tb3_ctx *ctx);
+
+/* Message of the last failed call on ctx (never NULL; "" after success).
+ * tb3_last_error(NULL) returns the message of the last failed tb3_create. */
+const char *tb3_last_error(const tb3_ctx *ctx);
+
+/* Replaces createNATAContext / createP3BCAContext / createAUTAContext
+ * (main.cpp:32-105) and the AxilrodTeller(nu) constructor
+ * (potential/AxilrodTeller.cpp:3).
+ *   algo      TB3_NATA, TB3_P3BCA or TB3_AUTA.  NATA and AUTA evaluate every
+ *             triplet (Algorithm.cpp:31); P3BCA applies the periodic cutoff
+ *             test (Algorithm.cpp:94-100) when cutoff > 0.
+ *   nu        Axilrod-Teller coefficient (the reference passes 1.0, main.cpp:41,66,91).
+ *   cutoff    P3BCA cutoff distance; <= 0 disables the test as in Algorithm.cpp:94.
+ *   domain_L  periodic domain size for the cutoff test, or NULL to take the
+ *             reference's rule at tb3_set_particles: the bounding box of the
+ *             initial particles widened by 1e-4 on each side
+ *             (RegularGridDecomposition.cpp:24-28, 135-166).
+ *   precision a tb3_precision value.
+ * May be called again at any time; a NULL domain_L recomputes the box from the
+ * particles currently held. */
+tb3_status tb3_configure(tb3_ctx *ctx, int algo, double nu, double cutoff, const double *domain_L, int precision);
+
+/* Replaces Utility::getParticlesFromCSV + the decomposition's copy-in
+ * (utility/utility.cpp:14-24; DomainDecomposition.cpp:7-8).  Host arrays,
+ * caller-owned, copied to the device: pos/vel/acc are n x 3 row-major (x,y,z
+ * per particle), mass is n, id is n (may be NULL: ids 0..n-1).  Forces are
+ * reset to zero.  Particle order is the sweep order, as the reference keeps
+ * the CSV row order (AtomDecomposition.cpp:34-36, RegularGridDecomposition.cpp:41-48). */
+tb3_status tb3_set_particles(tb3_ctx *ctx, int64_t n, const int32_t *id, const double *pos, const double *vel,
+                             const double *acc, const double *mass);
+
+/* Replaces DomainDecomposition::GetMyParticles (DomainDecomposition.cpp:7) as
+ * used by the CSV writer (Simulation.cpp:99-133).  Copies the current state
+ * back in the input order; any pointer may be NULL.  f is the summed force of
+ * the last sweep: +dE/dx, the reference's sign convention. */
+tb3_status tb3_get_particles(const tb3_ctx *ctx, int64_t n, int32_t *id, double *pos, double *vel, double *acc,
+                             double *mass, double *f);
+
+/* Replaces one iteration of Simulation::Start (Simulation.cpp:38-48):
+ * UpdatePredictorStage (structs.hpp:40-55) -> SimulationStep (ResetForces,
+ * sweep, SumUpParticles; Algorithm.cpp:72-139, 237-271) -> Update
+ * (structs.hpp:57-99).  *n_buffer receives the buffer-triplet count (1 at one
+ * GPU) and *n_triplets the accepted particle triplets (Algorithm.hpp:60);
+ * both may be NULL. */
+tb3_status tb3_step(tb3_ctx *ctx, double dt, const double g[3], uint64_t *n_buffer, uint64_t *n_triplets);
+
+/* `steps` iterations of tb3_step without host round trips between them
+ * (Simulation.cpp:30 loop).  counts (may be NULL) receives `steps` accepted
+ * triplet counts. */
+tb3_status tb3_run(tb3_ctx *ctx, int steps, double dt, const double g[3], uint64_t *counts);
+
+/* SimulationStep alone (no predictor or update): reset, sweep and role sum
+ * (P3BCA.cpp:516-805 / AUTA.cpp:236-383 at one process).  The force is read
+ * back with tb3_get_particles. */
+tb3_status tb3_compute_forces(tb3_ctx *ctx, uint64_t *n_triplets);
+
+/* Energy forward: E = sum over the accepted triplets of
+ * nu (1 + 3 cos g1 cos g2 cos g3) / (r12 r13 r23)^3 at the current positions.
+ * No counterpart in the reference (it never evaluates E); its gradient is the
+ * reference's f (AxilrodTeller.cpp:22-130). */
+tb3_status tb3_energy(tb3_ctx *ctx, double *E, uint64_t *n_triplets);
+
+/* Energy adjoint: E, dE/dx (= f, n x 3 host buffer, may be NULL) and
+ * dE/dnu = E / nu (may be NULL). */
+tb3_status tb3_energy_grad(tb3_ctx *ctx, double *E, double *dEdx, double *dEdnu, uint64_t *n_triplets);
+
+/* Device-pointer forms for framework bindings (torch).  d_pos is an n x 3
+ * row-major device array; results land in device memory; `stream` is a
+ * hipStream_t (NULL: the context's stream).  They use the context's
+ * algorithm, cutoff, domain and precision but not its particles, and leave
+ * the context's particle state untouched.  Asynchronous on `stream`, except
+ * that n_triplets (host, may be NULL) forces a synchronisation. */
+tb3_status tb3_energy_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, double nu, double *d_E, void *stream,
+                          uint64_t *n_triplets);
+tb3_status tb3_energy_grad_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, double nu, double *d_E,
+                               double *d_dEdx, void *stream, uint64_t *n_triplets);
+
+/* Per-phase timing with HIP events recorded on the context's stream around
+ * each phase of tb3_step / tb3_run / tb3_compute_forces.  No counterpart in
+ * the reference beyond its per-method timers (PROFILE_3BMDA, Algorithm.cpp:57-63,
+ * 145-162).  tb3_set_profiling(ctx, 1) enables and clears the accumulators.
+ * tb3_get_profile fills up to n values:
+ *   [0] steps timed, [1] predictor ms, [2] cell + neighbour-list ms,
+ *   [3] force kernel ms (the sweep kernel alone), [4] role sum + update ms,
+ *   [5] accepted triplets summed over the timed steps. */
+tb3_status tb3_set_profiling(tb3_ctx *ctx, int enable);
+tb3_status tb3_get_profile(const tb3_ctx *ctx, double *out, int64_t n);
+
+/* Diagnostic: element-wise sqrt(a) and a / b in device FP64, as the
+ * bit-exact path evaluates them, for checking that both round correctly.
+ * Host arrays of n doubles.  No counterpart in the reference. */
+tb3_status tb3_selftest_fp64(tb3_ctx *ctx, int64_t n, const double *a, const double *b, double *sqrt_a,
+                             double *a_div_b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,541 @@
+/*
+ * tb3_oracle.cpp -- CPU oracle (test infrastructure only; see tb3_oracle.h).
+ *
+ * Restates the reference's one-process hot path.  Compiled with
+ * -ffp-contract=off -fno-fast-math so that every expression rounds exactly as
+ * written; citations are relative to the reference's src/ directory.
+ */
+#include "tb3_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+/* 120-byte particle record, field for field as Utility::Particle
+ * (utility/structs.hpp:11-18).  The brute sweep runs over this AoS layout so
+ * the CPU baseline has the reference's memory shape. */
+struct Particle {
+    int ID;
+    double pX, pY, pZ;
+    double vX, vY, vZ;
+    double aX, aY, aZ;
+    double fX, fY, fZ;
+    double mass;
+    bool isDummy;
+};
+static_assert(sizeof(Particle) == 120, "reference particle record is 120 bytes");
+
+/* Eigen's Vector3d::norm() restated as sqrt((x*x + y*y) + z*z)
+ * (AxilrodTeller.cpp:29-31).  The summation order inside Eigen is not
+ * observable without the reference binary: parity unpinned at ulp level. */
+inline double norm3(double x, double y, double z) { return std::sqrt((x * x + y * y) + z * z); }
+
+/* The bracket of dVdRa/b/c (AxilrodTeller.cpp:68-81) written once for an own
+ * side x and the two other sides y, z in the order each of the three
+ * expressions names them:  a -> (x=a, y=b, z=c), b -> (x=b, y=a, z=c),
+ * c -> (x=c, y=b, z=a).  Products associate left to right as in the source. */
+inline double at_bracket(double nu, const double *dx, const double *dy, const double *dz)
+{
+    /* d?[0..5] hold d1..d6 */
+    const double d1x = dx[0], d2x = dx[1], d4x = dx[3], d6x = dx[5];
+    const double d1y = dy[0], d3y = dy[2], d5y = dy[4];
+    const double d1z = dz[0], d3z = dz[2], d5z = dz[4];
+    double t = -8. / (d4x * d3y * d3z);
+    t = t - 1. / (d5y * d5z);
+    t = t + 5. * d1y / (d6x * d5z);
+    t = t + 5. * d1z / (d6x * d5y);
+    t = t - 1. / (d2x * d3y * d5z);
+    t = t - 1. / (d2x * d5y * d3z);
+    t = t - 3. / (d4x * d1y * d5z);
+    t = t - 3. / (d4x * d5y * d1z);
+    t = t - 5. / (d6x * d1y * d3z);
+    t = t - 5. / (d6x * d3y * d1z);
+    t = t + 6. / (d4x * d3y * d3z);
+    return (3. * nu / (8. * d1x)) * t;
+}
+
+inline void powers(double d1, double *d)
+{
+    /* AxilrodTeller.cpp:33-51: d2 = d1*d1, d3 = d2*d1, ... d6 = d5*d1 */
+    d[0] = d1;
+    d[1] = d[0] * d1;
+    d[2] = d[1] * d1;
+    d[3] = d[2] * d1;
+    d[4] = d[3] * d1;
+    d[5] = d[4] * d1;
+}
+
+/* AxilrodTeller::CalculateForces body (AxilrodTeller.cpp:22-93). */
+inline void at_contrib_xyz(double xi, double yi, double zi, double xj, double yj, double zj, double xk, double yk,
+                           double zk, double nu, double *ci, double *cj, double *ck)
+{
+    /* sides a = rj - ri, b = ri - rk, c = rj - rk (23-25) */
+    const double d1a = norm3(xj - xi, yj - yi, zj - zi);
+    const double d1b = norm3(xi - xk, yi - yk, zi - zk);
+    const double d1c = norm3(xj - xk, yj - yk, zj - zk);
+    double da[6], db[6], dc[6];
+    powers(d1a, da);
+    powers(d1b, db);
+    powers(d1c, dc);
+
+    const double dXa = xj - xi, dYa = yj - yi, dZa = zj - zi; /* 53-55 */
+    const double dXb = xk - xi, dYb = yk - yi, dZb = zk - zi; /* 57-59 */
+    const double dXc = xk - xj, dYc = yk - yj, dZc = zk - zj; /* 61-63 */
+
+    const double dVdRa = at_bracket(nu, da, db, dc); /* 68-71 */
+    const double dVdRb = at_bracket(nu, db, da, dc); /* 73-76 */
+    const double dVdRc = at_bracket(nu, dc, db, da); /* 78-81 */
+
+    ci[0] = dXa * dVdRa + dXb * dVdRb; /* 83-85 */
+    ci[1] = dYa * dVdRa + dYb * dVdRb;
+    ci[2] = dZa * dVdRa + dZb * dVdRb;
+    cj[0] = dXa * (-dVdRa) + dXc * dVdRc; /* 87-89 */
+    cj[1] = dYa * (-dVdRa) + dYc * dVdRc;
+    cj[2] = dZa * (-dVdRa) + dZc * dVdRc;
+    ck[0] = dXb * (-dVdRb) + dXc * (-dVdRc); /* 91-93 */
+    ck[1] = dYb * (-dVdRb) + dYc * (-dVdRc);
+    ck[2] = dZb * (-dVdRb) + dZc * (-dVdRc);
+}
+
+/* GetSqrDistPeriodic (structs.hpp:106-115); std::min(a, b) is (b < a) ? b : a. */
+inline double sqr_dist_periodic(double ax, double ay, double az, double bx, double by, double bz, const double *L)
+{
+    const double absX = std::fabs(bx - ax);
+    const double absY = std::fabs(by - ay);
+    const double absZ = std::fabs(bz - az);
+    const double lx = L[0] - absX, ly = L[1] - absY, lz = L[2] - absZ;
+    const double xP = (lx < absX) ? lx : absX;
+    const double yP = (ly < absY) ? ly : absY;
+    const double zP = (lz < absZ) ? lz : absZ;
+    return xP * xP + yP * yP + zP * zP;
+}
+
+/* The per-triplet plug-in point, called through a virtual function as the
+ * reference calls Potential::CalculateForces (potential/Potential.hpp:25). */
+struct PotentialBase {
+    virtual ~PotentialBase() {}
+    virtual void CalculateForces(Particle &i, Particle &j, Particle &k) = 0;
+};
+struct AxilrodTellerOracle final : PotentialBase {
+    double nu;
+    explicit AxilrodTellerOracle(double v) : nu(v) {}
+    void CalculateForces(Particle &i, Particle &j, Particle &k) override
+    {
+        double ci[3], cj[3], ck[3];
+        at_contrib_xyz(i.pX, i.pY, i.pZ, j.pX, j.pY, j.pZ, k.pX, k.pY, k.pZ, nu, ci, cj, ck);
+        /* AxilrodTeller.cpp:119-129 */
+        i.fX -= ci[0];
+        i.fY -= ci[1];
+        i.fZ -= ci[2];
+        j.fX -= cj[0];
+        j.fY -= cj[1];
+        j.fZ -= cj[2];
+        k.fX -= ck[0];
+        k.fY -= ck[1];
+        k.fZ -= ck[2];
+    }
+};
+PotentialBase *volatile g_potential_slot = nullptr; /* keeps the call virtual */
+
+std::vector<Particle> make_buffer(int64_t n, const double *pos)
+{
+    std::vector<Particle> b((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        Particle &p = b[(size_t)i];
+        std::memset(&p, 0, sizeof(p));
+        p.ID = (int)i;
+        p.pX = pos[3 * i + 0];
+        p.pY = pos[3 * i + 1];
+        p.pZ = pos[3 * i + 2];
+        p.isDummy = false;
+    }
+    return b;
+}
+
+/* Algorithm::calculateInteractions loop nest (Algorithm.cpp:72-139) at one
+ * process: owners (0,0,0) so j starts at i+1 and k at j+1 (77-87). */
+uint64_t sweep_aos(std::vector<Particle> &b0, std::vector<Particle> &b1, std::vector<Particle> &b2, double cutoff,
+                   const double *L, int64_t row_end, uint64_t *possible_out, PotentialBase *pot)
+{
+    uint64_t act = 0, possible = 0;
+    const double sqrCutoff = cutoff * cutoff; /* 67 */
+    const size_t n = b0.size();
+    const size_t iend = row_end < 0 ? n : std::min(n, (size_t)row_end);
+    for (size_t i = 0; i < iend; ++i) {
+        if (b0[i].isDummy) continue;
+        for (size_t j = i + 1; j < b1.size(); ++j) {
+            if (b1[j].isDummy) continue;
+            for (size_t k = j + 1; k < b2.size(); ++k) {
+                if (b2[k].isDummy) continue;
+                possible++;
+                if (cutoff > 0) { /* 94-100 */
+                    if (sqr_dist_periodic(b0[i].pX, b0[i].pY, b0[i].pZ, b1[j].pX, b1[j].pY, b1[j].pZ, L) >
+                            sqrCutoff ||
+                        sqr_dist_periodic(b0[i].pX, b0[i].pY, b0[i].pZ, b2[k].pX, b2[k].pY, b2[k].pZ, L) >
+                            sqrCutoff ||
+                        sqr_dist_periodic(b1[j].pX, b1[j].pY, b1[j].pZ, b2[k].pX, b2[k].pY, b2[k].pZ, L) >
+                            sqrCutoff) {
+                        continue;
+                    }
+                }
+                pot->CalculateForces(b0[i], b1[j], b2[k]); /* 109 */
+                act++;                                      /* 138 */
+            }
+        }
+    }
+    if (possible_out) *possible_out = possible;
+    return act;
+}
+
+int resolve_threads(int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    return nthreads;
+#else
+    (void)nthreads;
+    return 1;
+#endif
+}
+
+/* Index-sorted neighbour lists: q != p with !(d2(p,q) > c^2) when cutoff > 0
+ * (the negated test of Algorithm.cpp:95-97), every q != p otherwise. */
+void neighbour_lists(int64_t n, const double *pos, double cutoff, const double *L,
+                     std::vector<std::vector<int>> &nb, int nthreads)
+{
+    nb.assign((size_t)n, std::vector<int>());
+    const double c2 = cutoff * cutoff;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int64_t p = 0; p < n; ++p) {
+        std::vector<int> &l = nb[(size_t)p];
+        for (int64_t q = 0; q < n; ++q) {
+            if (q == p) continue;
+            if (cutoff > 0) {
+                const double d2 = sqr_dist_periodic(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], pos[3 * q],
+                                                    pos[3 * q + 1], pos[3 * q + 2], L);
+                if (d2 > c2) continue;
+            }
+            l.push_back((int)q);
+        }
+    }
+}
+
+inline bool pair_ok(const double *pos, int a, int b, double cutoff, double c2, const double *L)
+{
+    if (!(cutoff > 0)) return true;
+    return !(sqr_dist_periodic(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], pos[3 * b], pos[3 * b + 1],
+                               pos[3 * b + 2], L) > c2);
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_at_contrib(const double *pi, const double *pj, const double *pk, double nu, double *ci, double *cj,
+                    double *ck)
+{
+    at_contrib_xyz(pi[0], pi[1], pi[2], pj[0], pj[1], pj[2], pk[0], pk[1], pk[2], nu, ci, cj, ck);
+}
+
+double orc_sqr_dist_periodic(const double *pa, const double *pb, const double *L)
+{
+    return sqr_dist_periodic(pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], L);
+}
+
+uint64_t orc_sweep_brute_rows(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f0,
+                              double *f1, double *f2, int64_t rows, uint64_t *accepted)
+{
+    /* ResetForces then b0 = copy, b1 = b0, b2 = b0 (P3BCA.cpp:518-538; AUTA.cpp:239-247) */
+    std::vector<Particle> b0 = make_buffer(n, pos);
+    std::vector<Particle> b1 = b0;
+    std::vector<Particle> b2 = b0;
+    AxilrodTellerOracle at(nu);
+    g_potential_slot = &at;
+    const double Lz[3] = {-1, -1, -1};
+    uint64_t possible = 0;
+    uint64_t act = sweep_aos(b0, b1, b2, cutoff, L ? L : Lz, rows, &possible, g_potential_slot);
+    for (int64_t i = 0; i < n; ++i) {
+        const Particle &p0 = b0[(size_t)i], &p1 = b1[(size_t)i], &p2 = b2[(size_t)i];
+        if (f0) {
+            f0[3 * i] = p0.fX;
+            f0[3 * i + 1] = p0.fY;
+            f0[3 * i + 2] = p0.fZ;
+        }
+        if (f1) {
+            f1[3 * i] = p1.fX;
+            f1[3 * i + 1] = p1.fY;
+            f1[3 * i + 2] = p1.fZ;
+        }
+        if (f2) {
+            f2[3 * i] = p2.fX;
+            f2[3 * i + 1] = p2.fY;
+            f2[3 * i + 2] = p2.fZ;
+        }
+    }
+    if (accepted) *accepted = act;
+    return possible;
+}
+
+uint64_t orc_sweep_brute(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f0,
+                         double *f1, double *f2, uint64_t *possible)
+{
+    uint64_t act = 0;
+    uint64_t poss = orc_sweep_brute_rows(n, pos, nu, cutoff, L, f0, f1, f2, -1, &act);
+    if (possible) *possible = poss;
+    return act;
+}
+
+uint64_t orc_sweep_owner(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f0,
+                         double *f1, double *f2, int nthreads)
+{
+    nthreads = resolve_threads(nthreads);
+    const double Lz[3] = {-1, -1, -1};
+    if (!L) L = Lz;
+    std::vector<std::vector<int>> nb;
+    neighbour_lists(n, pos, cutoff, L, nb, nthreads);
+    const double c2 = cutoff * cutoff;
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total) num_threads(nthreads)
+    for (int64_t p = 0; p < n; ++p) {
+        const std::vector<int> &l = nb[(size_t)p];
+        /* split the sorted list into N-(p) = {q < p} and N+(p) = {q > p} */
+        const size_t split = (size_t)(std::lower_bound(l.begin(), l.end(), (int)p) - l.begin());
+        double a0[3] = {0.0, 0.0, 0.0}, a1[3] = {0.0, 0.0, 0.0}, a2[3] = {0.0, 0.0, 0.0};
+        double ci[3], cj[3], ck[3];
+        const double *P = pos + 3 * p;
+        uint64_t cnt = 0;
+        /* role 0: p = i, (j,k) lexicographic over N+ */
+        for (size_t a = split; a < l.size(); ++a) {
+            const int j = l[a];
+            for (size_t b = a + 1; b < l.size(); ++b) {
+                const int k = l[b];
+                if (!pair_ok(pos, j, k, cutoff, c2, L)) continue;
+                at_contrib_xyz(P[0], P[1], P[2], pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], pos[3 * k],
+                               pos[3 * k + 1], pos[3 * k + 2], nu, ci, cj, ck);
+                a0[0] -= ci[0];
+                a0[1] -= ci[1];
+                a0[2] -= ci[2];
+                cnt++;
+            }
+        }
+        /* role 1: p = j, (i,k) lexicographic, i in N-, k in N+ */
+        for (size_t a = 0; a < split; ++a) {
+            const int i = l[a];
+            for (size_t b = split; b < l.size(); ++b) {
+                const int k = l[b];
+                if (!pair_ok(pos, i, k, cutoff, c2, L)) continue;
+                at_contrib_xyz(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], P[0], P[1], P[2], pos[3 * k],
+                               pos[3 * k + 1], pos[3 * k + 2], nu, ci, cj, ck);
+                a1[0] -= cj[0];
+                a1[1] -= cj[1];
+                a1[2] -= cj[2];
+            }
+        }
+        /* role 2: p = k, (i,j) lexicographic over N- */
+        for (size_t a = 0; a < split; ++a) {
+            const int i = l[a];
+            for (size_t b = a + 1; b < split; ++b) {
+                const int j = l[b];
+                if (!pair_ok(pos, i, j, cutoff, c2, L)) continue;
+                at_contrib_xyz(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], pos[3 * j], pos[3 * j + 1],
+                               pos[3 * j + 2], P[0], P[1], P[2], nu, ci, cj, ck);
+                a2[0] -= ck[0];
+                a2[1] -= ck[1];
+                a2[2] -= ck[2];
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            f0[3 * p + c] = a0[c];
+            f1[3 * p + c] = a1[c];
+            f2[3 * p + c] = a2[c];
+        }
+        total += cnt;
+    }
+    return total;
+}
+
+void orc_sum_roles(int64_t n, const double *f0, const double *f1, const double *f2, double *f)
+{
+    for (int64_t i = 0; i < 3 * n; ++i) f[i] = f0[i] + (f1[i] + f2[i]); /* Algorithm.cpp:246-248 */
+}
+
+void orc_predictor(int64_t n, double *pos, const double *vel, const double *acc, double dt)
+{
+    const double h = dt * dt * 0.5; /* structs.hpp:45 */
+    for (int64_t i = 0; i < n; ++i) {
+        const double aX = acc[3 * i]; /* structs.hpp:44: acc(aX, aX, aX) */
+        pos[3 * i + 0] = pos[3 * i + 0] + vel[3 * i + 0] * dt + aX * h;
+        pos[3 * i + 1] = pos[3 * i + 1] + vel[3 * i + 1] * dt + aX * h;
+        pos[3 * i + 2] = pos[3 * i + 2] + vel[3 * i + 2] * dt + aX * h;
+    }
+}
+
+void orc_update(int64_t n, const double *f, const double *mass, double *vel, double *acc, double dt,
+                const double *g)
+{
+    const double h = dt * 0.5; /* structs.hpp:75 */
+    for (int64_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            const double dragAcc = f[3 * i + c] / mass[i]; /* 69 */
+            const double newAcc = g[c] - dragAcc;         /* 71 */
+            vel[3 * i + c] = vel[3 * i + c] + newAcc * h;  /* 75, 92-94 */
+            acc[3 * i + c] = newAcc;                       /* 96-98 */
+        }
+    }
+}
+
+void orc_domain_size(int64_t n, const double *pos, double *L, double *lo)
+{
+    double mn[3], mx[3];
+    for (int c = 0; c < 3; ++c) {
+        mn[c] = 1.7976931348623157e308;   /* numeric_limits<double>::max() */
+        mx[c] = -1.7976931348623157e308;  /* numeric_limits<double>::lowest() */
+    }
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const double v = pos[3 * i + c];
+            if (v < mn[c]) mn[c] = v;
+            if (v > mx[c]) mx[c] = v;
+        }
+    for (int c = 0; c < 3; ++c) {
+        const double a = mn[c] - 0.0001, b = mx[c] + 0.0001;
+        L[c] = std::fabs(b - a);
+        if (lo) lo[c] = a;
+    }
+}
+
+int orc_simulate(int64_t n, double *pos, double *vel, double *acc, const double *mass, double *f, int algo,
+                 int steps, double dt, const double *g, double cutoff, double nu, uint64_t *counts, int mode,
+                 int nthreads, orc_step_cb step_cb, void *user)
+{
+    double L[3] = {-1, -1, -1};
+    double cut = -1.0;
+    if (algo == 1) {
+        orc_domain_size(n, pos, L, nullptr);
+        cut = cutoff;
+    }
+    std::vector<double> f0((size_t)(3 * n)), f1((size_t)(3 * n)), f2((size_t)(3 * n));
+    for (int s = 0; s < steps; ++s) {
+        orc_predictor(n, pos, vel, acc, dt);
+        uint64_t c;
+        if (mode == 0)
+            c = orc_sweep_brute(n, pos, nu, cut, L, f0.data(), f1.data(), f2.data(), nullptr);
+        else
+            c = orc_sweep_owner(n, pos, nu, cut, L, f0.data(), f1.data(), f2.data(), nthreads);
+        orc_sum_roles(n, f0.data(), f1.data(), f2.data(), f);
+        orc_update(n, f, mass, vel, acc, dt, g);
+        if (counts) counts[s] = c;
+        if (step_cb) step_cb(s, user);
+    }
+    return 0;
+}
+
+double orc_energy(int64_t n, const double *pos, double nu, double cutoff, const double *L, uint64_t *count,
+                  int nthreads)
+{
+    nthreads = resolve_threads(nthreads);
+    const double Lz[3] = {-1, -1, -1};
+    if (!L) L = Lz;
+    std::vector<std::vector<int>> nb;
+    neighbour_lists(n, pos, cutoff, L, nb, nthreads);
+    const double c2 = cutoff * cutoff;
+    double E = 0.0;
+    uint64_t cnt = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : E, cnt) num_threads(nthreads)
+    for (int64_t p = 0; p < n; ++p) {
+        const std::vector<int> &l = nb[(size_t)p];
+        const size_t split = (size_t)(std::lower_bound(l.begin(), l.end(), (int)p) - l.begin());
+        const double *P = pos + 3 * p;
+        for (size_t a = split; a < l.size(); ++a) {
+            const int j = l[a];
+            for (size_t b = a + 1; b < l.size(); ++b) {
+                const int k = l[b];
+                if (!pair_ok(pos, j, k, cutoff, c2, L)) continue;
+                const double *J = pos + 3 * j, *K = pos + 3 * k;
+                /* edge vectors and the three interior angles */
+                const double ij[3] = {J[0] - P[0], J[1] - P[1], J[2] - P[2]};
+                const double ik[3] = {K[0] - P[0], K[1] - P[1], K[2] - P[2]};
+                const double jk[3] = {K[0] - J[0], K[1] - J[1], K[2] - J[2]};
+                const double rij = std::sqrt(ij[0] * ij[0] + ij[1] * ij[1] + ij[2] * ij[2]);
+                const double rik = std::sqrt(ik[0] * ik[0] + ik[1] * ik[1] + ik[2] * ik[2]);
+                const double rjk = std::sqrt(jk[0] * jk[0] + jk[1] * jk[1] + jk[2] * jk[2]);
+                const double cosi = (ij[0] * ik[0] + ij[1] * ik[1] + ij[2] * ik[2]) / (rij * rik);
+                const double cosj = (-ij[0] * jk[0] - ij[1] * jk[1] - ij[2] * jk[2]) / (rij * rjk);
+                const double cosk = (ik[0] * jk[0] + ik[1] * jk[1] + ik[2] * jk[2]) / (rik * rjk);
+                const double r = rij * rik * rjk;
+                E += nu * (1.0 + 3.0 * cosi * cosj * cosk) / (r * r * r);
+                cnt++;
+            }
+        }
+    }
+    if (count) *count = cnt;
+    return E;
+}
+
+double orc_forces_fast(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f,
+                       uint64_t *count, int nthreads)
+{
+    nthreads = resolve_threads(nthreads);
+    const double Lz[3] = {-1, -1, -1};
+    if (!L) L = Lz;
+    std::vector<std::vector<int>> nb;
+    neighbour_lists(n, pos, cutoff, L, nb, nthreads);
+    const double c2 = cutoff * cutoff;
+    const double K = 0.375 * nu;
+    double E = 0.0;
+    uint64_t cnt = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : E, cnt) num_threads(nthreads)
+    for (int64_t p = 0; p < n; ++p) {
+        const std::vector<int> &l = nb[(size_t)p];
+        const double *P = pos + 3 * p;
+        double g[3] = {0, 0, 0};
+        for (size_t a = 0; a < l.size(); ++a) {
+            const int j = l[a];
+            const double *J = pos + 3 * j;
+            const double dj[3] = {J[0] - P[0], J[1] - P[1], J[2] - P[2]};
+            const double u = dj[0] * dj[0] + dj[1] * dj[1] + dj[2] * dj[2];
+            for (size_t b = a + 1; b < l.size(); ++b) {
+                const int k = l[b];
+                if (!pair_ok(pos, j, k, cutoff, c2, L)) continue;
+                const double *Kp = pos + 3 * k;
+                const double dk[3] = {Kp[0] - P[0], Kp[1] - P[1], Kp[2] - P[2]};
+                const double djk[3] = {Kp[0] - J[0], Kp[1] - J[1], Kp[2] - J[2]};
+                const double v = dk[0] * dk[0] + dk[1] * dk[1] + dk[2] * dk[2];
+                const double w = djk[0] * djk[0] + djk[1] * djk[1] + djk[2] * djk[2];
+                /* E = nu s^3 + K A B C s^5, s = (uvw)^-1/2, A = u+v-w, B = u-v+w, C = -u+v+w */
+                const double s = 1.0 / std::sqrt(u * v * w);
+                const double s2 = s * s;
+                const double A = u + v - w, B = u - v + w, C = -u + v + w;
+                const double X = B * C, Y = A * C, Z = A * B;
+                const double ABC = A * X;
+                const double Q = ABC * s2;
+                const double R = -1.5 * nu - 2.5 * K * Q;
+                const double S = X + Y + Z;
+                const double s5 = s2 * s2 * s;
+                const double Eu = s5 * (R * v * w + K * (S - 2.0 * Z));
+                const double Ev = s5 * (R * u * w + K * (S - 2.0 * Y));
+                /* dE/dx_p = -2 Eu (xj - xp) - 2 Ev (xk - xp) */
+                for (int c = 0; c < 3; ++c) g[c] += -2.0 * Eu * dj[c] - 2.0 * Ev * dk[c];
+                if (p < j && p < k) {
+                    E += s2 * s * (nu + K * Q);
+                    cnt++;
+                }
+            }
+        }
+        f[3 * p] = g[0];
+        f[3 * p + 1] = g[1];
+        f[3 * p + 2] = g[2];
+    }
+    if (count) *count = cnt;
+    return E;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+"""ctypes binding to the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker: tests call it on the same inputs they hand to the
+HIP library and compare.  Nothing in the product imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+ORACLE_CLI = os.path.join(ORACLE_DIR, "tb3_oracle_cli")
+
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def build():
+    """Compile the oracle if the shared library or CLI is missing or stale."""
+    srcs = [os.path.join(ORACLE_DIR, f) for f in ("tb3_oracle.cpp", "tb3_oracle.h", "tb3_oracle_main.cpp")]
+    outs = [ORACLE_SO, ORACLE_CLI]
+    newest = max(os.path.getmtime(s) for s in srcs)
+    if all(os.path.exists(o) and os.path.getmtime(o) >= newest for o in outs):
+        return
+    subprocess.run(["make", "-C", ORACLE_DIR, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_at_contrib.argtypes = [_dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_sqr_dist_periodic.argtypes = [_dp, _dp, _dp]
+        L.orc_sqr_dist_periodic.restype = ctypes.c_double
+        L.orc_sweep_brute.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp,
+                                      _u64p]
+        L.orc_sweep_brute.restype = ctypes.c_uint64
+        L.orc_sweep_owner.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp,
+                                      ctypes.c_int]
+        L.orc_sweep_owner.restype = ctypes.c_uint64
+        L.orc_sum_roles.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp]
+        L.orc_predictor.argtypes = [ctypes.c_int64, _dp, _dp, _dp, ctypes.c_double]
+        L.orc_update.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_domain_size.argtypes = [ctypes.c_int64, _dp, _dp, _dp]
+        L.orc_simulate.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double, _u64p, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_energy.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _u64p, ctypes.c_int]
+        L.orc_energy.restype = ctypes.c_double
+        L.orc_forces_fast.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _u64p,
+                                      ctypes.c_int]
+        L.orc_forces_fast.restype = ctypes.c_double
+        L.orc_sweep_brute_rows.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
+                                           _dp, ctypes.c_int64, _u64p]
+        L.orc_sweep_brute_rows.restype = ctypes.c_uint64
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def at_contrib(pi, pj, pk, nu=1.0):
+    """Contributions the reference subtracts from i.f, j.f, k.f for one triplet."""
+    ci, cj, ck = np.zeros(3), np.zeros(3), np.zeros(3)
+    lib().orc_at_contrib(_p(_c(pi)), _p(_c(pj)), _p(_c(pk)), nu, _p(ci), _p(cj), _p(ck))
+    return ci, cj, ck
+
+
+def sqr_dist_periodic(a, b, L):
+    return lib().orc_sqr_dist_periodic(_p(_c(a)), _p(_c(b)), _p(_c(L)))
+
+
+def sweep(pos, nu=1.0, cutoff=-1.0, L=None, mode="owner", threads=0):
+    """One force sweep: returns (f0, f1, f2, f, count) with f = f0 + (f1 + f2)."""
+    pos = _c(pos)
+    n = pos.shape[0]
+    f0, f1, f2, f = (np.zeros((n, 3)) for _ in range(4))
+    Lp = _p(_c(L)) if L is not None else None
+    if mode == "brute":
+        poss = ctypes.c_uint64(0)
+        cnt = lib().orc_sweep_brute(n, _p(pos), nu, cutoff, Lp, _p(f0), _p(f1), _p(f2), ctypes.byref(poss))
+    else:
+        cnt = lib().orc_sweep_owner(n, _p(pos), nu, cutoff, Lp, _p(f0), _p(f1), _p(f2), threads)
+    lib().orc_sum_roles(n, _p(f0), _p(f1), _p(f2), _p(f))
+    return f0, f1, f2, f, int(cnt)
+
+
+def brute_possible(pos, cutoff=-1.0, L=None, nu=1.0):
+    pos = _c(pos)
+    n = pos.shape[0]
+    f0, f1, f2 = (np.zeros((n, 3)) for _ in range(3))
+    poss = ctypes.c_uint64(0)
+    Lp = _p(_c(L)) if L is not None else None
+    act = lib().orc_sweep_brute(n, _p(pos), nu, cutoff, Lp, _p(f0), _p(f1), _p(f2), ctypes.byref(poss))
+    return int(act), int(poss.value)
+
+
+def domain_size(pos):
+    pos = _c(pos)
+    L, lo = np.zeros(3), np.zeros(3)
+    lib().orc_domain_size(pos.shape[0], _p(pos), _p(L), _p(lo))
+    return L, lo
+
+
+def predictor(pos, vel, acc, dt):
+    pos = _c(pos).copy()
+    lib().orc_predictor(pos.shape[0], _p(pos), _p(_c(vel)), _p(_c(acc)), dt)
+    return pos
+
+
+def update(f, mass, vel, acc, dt, g):
+    vel = _c(vel).copy()
+    acc = _c(acc).copy()
+    lib().orc_update(vel.shape[0], _p(_c(f)), _p(_c(mass)), _p(vel), _p(acc), dt, _p(_c(g)))
+    return vel, acc
+
+
+ALGO = {"nata": 0, "p3bca": 1, "auta": 2}
+
+
+def simulate(pos, vel, acc, mass, algo, steps, dt, g, cutoff=0.0, nu=1.0, mode="owner", threads=0):
+    """Simulation::Start at one process; returns (pos, vel, acc, f, counts)."""
+    pos, vel, acc = _c(pos).copy(), _c(vel).copy(), _c(acc).copy()
+    mass = _c(mass)
+    n = pos.shape[0]
+    f = np.zeros((n, 3))
+    counts = np.zeros(max(steps, 1), dtype=np.uint64)
+    lib().orc_simulate(n, _p(pos), _p(vel), _p(acc), _p(mass), _p(f), ALGO[algo] if isinstance(algo, str) else algo,
+                       steps, dt, _p(_c(g)), cutoff, nu, counts.ctypes.data_as(_u64p), 0 if mode == "brute" else 1,
+                       threads, None, None)
+    return pos, vel, acc, f, counts[:steps]
+
+
+def energy(pos, nu=1.0, cutoff=-1.0, L=None, threads=0):
+    pos = _c(pos)
+    cnt = ctypes.c_uint64(0)
+    Lp = _p(_c(L)) if L is not None else None
+    E = lib().orc_energy(pos.shape[0], _p(pos), nu, cutoff, Lp, ctypes.byref(cnt), threads)
+    return E, int(cnt.value)
+
+
+def forces_fast(pos, nu=1.0, cutoff=-1.0, L=None, threads=0):
+    pos = _c(pos)
+    f = np.zeros_like(pos)
+    cnt = ctypes.c_uint64(0)
+    Lp = _p(_c(L)) if L is not None else None
+    E = lib().orc_forces_fast(pos.shape[0], _p(pos), nu, cutoff, Lp, _p(f), ctypes.byref(cnt), threads)
+    return f, E, int(cnt.value)

@@ -1,0 +1,194 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle on
+the same seeded inputs.
+
+Bars (written into each test):
+  * TB3_FP64_EXACT: bit-identical forces, positions, velocities, accelerations
+    and triplet counts, over every step.
+  * TB3_FP64 (reformulated, reordered): one sweep from identical positions,
+    per-particle |df| <= 1e-12 max(|f|, rms|f|) and RMS of |df|/|f| <= 1e-12;
+    counts bit-exact.  (A particle whose net force nearly cancels amplifies
+    ulp-level differences in the pure ratio |df|/|f|, which is held to 1e-10.)
+  * TB3_FP32_MIXED: one sweep, |df| <= 1e-5 max(|f|, rms|f|), RMS |df|/|f|
+    <= 1e-5; counts bit-exact (the cutoff test is FP64).  bench.py states the
+    measured errors.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import threebodymdalgorithms_amd as tb3
+from conftest import clustered_particles, uniform_particles
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = tb3.Engine(0)
+    yield e
+    e.close()
+
+
+def per_particle_rel(a, b):
+    num = np.linalg.norm(a - b, axis=1)
+    den = np.linalg.norm(b, axis=1)
+    return np.max(num / np.where(den > 0, den, 1.0))
+
+
+def force_errors(f, fo):
+    """Per-particle force errors of f against the reference fo: pure ratio
+    |df|/|F|, its RMS, and the ratio with |F| floored at the RMS force."""
+    num = np.linalg.norm(f - fo, axis=1)
+    den = np.linalg.norm(fo, axis=1)
+    rel = num / np.where(den > 0, den, 1.0)
+    floor = np.sqrt(np.mean(den ** 2))
+    return {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel ** 2))),
+            "max_rel_floored": float((num / np.maximum(den, floor)).max())}
+
+
+def gpu_forces(eng, P, algo, cutoff, precision, nu=1.0, L=None):
+    eng.set_particles(P)
+    eng.configure(algo, nu=nu, cutoff=cutoff, domain_L=L, precision=precision)
+    cnt = eng.compute_forces()
+    return eng.get_particles()["f"], cnt
+
+
+def test_fp64_sqrt_and_division_round_correctly(eng):
+    """The exact path needs IEEE-rounded FP64 sqrt and division on the device."""
+    rng = np.random.default_rng(0)
+    n = 2_000_000
+    a = np.concatenate([rng.uniform(0, 1, n // 4), 10.0 ** rng.uniform(-30, 30, n // 4),
+                        rng.uniform(1, 4, n // 4), np.abs(rng.normal(size=n // 4)) * 1e8])
+    b = np.concatenate([rng.uniform(0.5, 2, n // 2), 10.0 ** rng.uniform(-20, 20, n // 2)]) * \
+        rng.choice([-1.0, 1.0], n)
+    say(f"selftest_fp64 n={n}")
+    sq, q = eng.selftest_fp64(a, b)
+    assert np.array_equal(sq, np.sqrt(a)), int(np.sum(sq != np.sqrt(a)))
+    assert np.array_equal(q, a / b), int(np.sum(q != a / b))
+
+
+@pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 150, 4.0, 0.0), ("nata", 97, 4.0, 0.0),
+                                               ("p3bca", 800, 9.0, 2.0), ("p3bca", 300, 5.0, 0.0),
+                                               ("p3bca", 400, 4.5, 2.4)])
+def test_exact_forces_bit_identical(eng, oracle, algo, n, box, cutoff):
+    P = uniform_particles(n, box, 100 + n)
+    L, _ = oracle.domain_size(P)
+    use_cut = cutoff if algo == "p3bca" else -1.0
+    say(f"exact {algo} n={n} cutoff={cutoff}")
+    f, cnt = gpu_forces(eng, P, algo, cutoff, "exact")
+    *_, fo, co = oracle.sweep(P, cutoff=use_cut, L=L, mode="owner")
+    assert cnt == co
+    assert np.array_equal(f, fo), per_particle_rel(f, fo)
+
+
+def test_exact_forces_particles_outside_box(eng, oracle):
+    """Particles displaced by one or two box lengths (the one-image rule)."""
+    n = 500
+    P = uniform_particles(n, 8.0, 5)
+    L, _ = oracle.domain_size(P)
+    rng = np.random.default_rng(6)
+    idx = rng.choice(n, 60, replace=False)
+    P2 = P.copy()
+    P2[idx, rng.integers(0, 3, 60)] += L[0] * rng.choice([-2.0, -1.0, 1.0, 2.0], 60)
+    say("exact p3bca outside-box")
+    f, cnt = gpu_forces(eng, P2, "p3bca", 2.0, "exact", L=L)
+    *_, fo, co = oracle.sweep(P2, cutoff=2.0, L=L, mode="owner")
+    assert cnt == co
+    assert np.array_equal(f, fo)
+
+
+def test_exact_trajectory_bit_identical(eng, oracle):
+    """10 steps of P3BCA (c = 2, density 2): every state array and every count."""
+    n = 3000
+    box = (n / 2.0) ** (1 / 3)
+    P = uniform_particles(n, box, 77)
+    V, A, M = np.zeros_like(P), np.zeros_like(P), np.ones(n)
+    g = np.array([0.0, -9.81, 0.0])
+    eng.set_particles(P, V, A, M)
+    eng.configure("p3bca", cutoff=2.0, precision="exact")
+    say("exact trajectory p3bca n=3000 x10")
+    counts = [eng.step(0.001, g)[1] for _ in range(10)]
+    st = eng.get_particles()
+    pos, vel, acc, f, oc = oracle.simulate(P, V, A, M, "p3bca", 10, 0.001, g, cutoff=2.0)
+    assert counts == [int(c) for c in oc]
+    for name, a, b in (("pos", st["pos"], pos), ("vel", st["vel"], vel), ("acc", st["acc"], acc),
+                       ("f", st["f"], f)):
+        assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0)])
+def test_fast_fp64_one_sweep(eng, oracle, algo, n, box, cutoff):
+    P = uniform_particles(n, box, 300 + n)
+    L, _ = oracle.domain_size(P)
+    say(f"fp64 {algo} n={n}")
+    f, cnt = gpu_forces(eng, P, algo, cutoff, "fp64")
+    *_, fo, co = oracle.sweep(P, cutoff=cutoff if algo == "p3bca" else -1.0, L=L, mode="owner")
+    assert cnt == co
+    e = force_errors(f, fo)
+    say(f"  fp64 errors {e}")
+    assert e["max_rel_floored"] <= 1e-12 and e["rms_rel"] <= 1e-12
+    assert e["max_rel"] <= 1e-10
+
+
+@pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0)])
+def test_fp32_mixed_one_sweep(eng, oracle, algo, n, box, cutoff):
+    P = uniform_particles(n, box, 500 + n)
+    L, _ = oracle.domain_size(P)
+    say(f"fp32 {algo} n={n}")
+    f, cnt = gpu_forces(eng, P, algo, cutoff, "fp32")
+    *_, fo, co = oracle.sweep(P, cutoff=cutoff if algo == "p3bca" else -1.0, L=L, mode="owner")
+    assert cnt == co
+    e = force_errors(f, fo)
+    say(f"  fp32 errors {e}")
+    # FP32 rounding of u, v, w (~6e-8 each) bounds each triplet to ~1e-7;
+    # particles whose net force nearly cancels amplify that in the pure
+    # per-particle ratio, so the floored ratio (floor = RMS |F|) carries the bar.
+    assert e["max_rel_floored"] <= 1e-5
+    assert e["rms_rel"] <= 1e-5
+    assert e["max_rel"] <= 1e-3
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp64", "fp32"])
+def test_energy_and_adjoint(eng, oracle, precision):
+    n = 1500
+    P = clustered_particles(n, 6, 10.0, 1.0, 9)
+    L, _ = oracle.domain_size(P)
+    eng.set_particles(P)
+    eng.configure("p3bca", nu=1.3, cutoff=2.5, precision=precision)
+    say(f"energy {precision}")
+    E, cnt = eng.energy()
+    E2, g, dnu, cnt2 = eng.energy_grad()
+    Eo, co = oracle.energy(P, nu=1.3, cutoff=2.5, L=L)
+    *_, fo, _ = oracle.sweep(P, nu=1.3, cutoff=2.5, L=L, mode="owner")
+    assert cnt == cnt2 == co
+    tol = 1e-6 if precision == "fp32" else 1e-12
+    assert E == pytest.approx(Eo, rel=tol)
+    assert E2 == pytest.approx(Eo, rel=tol)
+    assert dnu == pytest.approx(E2 / 1.3, rel=1e-15)
+    if precision == "exact":
+        assert np.array_equal(g, fo)
+    else:
+        e = force_errors(g, fo)
+        say(f"  {precision} adjoint errors {e}")
+        if precision == "fp32":
+            assert e["max_rel_floored"] <= 1e-5 and e["rms_rel"] <= 1e-5
+        else:
+            assert e["max_rel_floored"] <= 1e-12 and e["rms_rel"] <= 1e-12
+            assert e["max_rel"] <= 1e-10
+
+
+def test_empty_and_tiny_systems(eng):
+    for n in (0, 1, 2):
+        P = np.arange(3 * n, dtype=float).reshape(n, 3)
+        eng.set_particles(P)
+        eng.configure("p3bca", cutoff=2.0, domain_L=[10, 10, 10], precision="exact")
+        nb, nt = eng.step(0.001, [0, -9.81, 0])
+        assert nt == 0
+        st = eng.get_particles()
+        assert np.array_equal(st["f"], np.zeros((n, 3)))

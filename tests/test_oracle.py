@@ -1,0 +1,257 @@
+"""Pins the CPU oracle before it is trusted as the parity checker.
+
+The reference ships no force or energy golden values (its tests pin counts
+only, tests.cpp:530-568), so each oracle function is checked against what the
+mathematics fixes: closed forms, the gradient of an independently coded
+energy, Newton's third law, invariances, exact linearity in nu, the
+reference's own C(N,3) known answer, and bit-identity between the brute
+i<j<k sweep and the owner-ordered sweep the GPU exact path reproduces.
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import uniform_particles
+
+
+def rot(seed):
+    rng = np.random.default_rng(seed)
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.diag(r))
+
+
+# --- per-triplet expression (AxilrodTeller.cpp:22-93) -------------------------
+
+@pytest.mark.parametrize("a,nu", [(1.0, 1.0), (1.3, 1.0), (0.7, 2.5), (2.2, 0.3)])
+def test_equilateral_closed_form(oracle, a, nu):
+    P = np.array([[0, 0, 0], [a, 0, 0], [a / 2, a * math.sqrt(3) / 2, 0]], float)
+    P = P @ rot(7).T + np.array([3.0, -1.0, 2.0])
+    E, cnt = oracle.energy(P, nu=nu)
+    assert cnt == 1
+    assert E == pytest.approx(11 * nu / (8 * a ** 9), rel=1e-13)
+    *_, f, c = oracle.sweep(P, nu=nu, mode="brute")
+    centroid = P.mean(axis=0)
+    mag = 33 * math.sqrt(3) * nu / (8 * a ** 10)
+    for i in range(3):
+        # f = +dE/dx points from the particle towards the centroid
+        u = (centroid - P[i]) / np.linalg.norm(centroid - P[i])
+        assert np.allclose(f[i], mag * u, rtol=1e-12, atol=1e-12 * mag)
+
+
+@pytest.mark.parametrize("a,nu", [(1.0, 1.0), (0.9, 1.7)])
+def test_collinear_closed_form(oracle, a, nu):
+    P = np.array([[0, 0, 0], [a, 0, 0], [2 * a, 0, 0]], float)
+    E, _ = oracle.energy(P, nu=nu)
+    assert E == pytest.approx(-nu / (4 * a ** 9), rel=1e-13)
+    *_, f, _ = oracle.sweep(P, nu=nu, mode="brute")
+    end = 9 * nu / (8 * a ** 10)
+    assert f[0, 0] == pytest.approx(-end, rel=1e-13)
+    assert f[2, 0] == pytest.approx(end, rel=1e-13)
+    assert abs(f[1, 0]) < 1e-13 * end
+    assert np.all(np.abs(f[:, 1:]) < 1e-13 * end)
+
+
+def test_gradient_is_energy_derivative(oracle):
+    """f equals the central-difference gradient of the textbook energy."""
+    rng = np.random.default_rng(11)
+    for trial in range(20):
+        P = rng.uniform(0, 2.0, size=(3, 3))
+        ci, cj, ck = oracle.at_contrib(P[0], P[1], P[2])
+        f = -np.array([ci, cj, ck])  # the reference subtracts the contributions
+        h = 1e-6
+        g = np.zeros((3, 3))
+        for i in range(3):
+            for c in range(3):
+                Pp, Pm = P.copy(), P.copy()
+                Pp[i, c] += h
+                Pm[i, c] -= h
+                g[i, c] = (oracle.energy(Pp)[0] - oracle.energy(Pm)[0]) / (2 * h)
+        scale = np.abs(f).max()
+        assert np.abs(g - f).max() <= 2e-7 * scale, trial
+
+
+def test_newton_third_law_and_invariance(oracle):
+    rng = np.random.default_rng(3)
+    R = rot(5)
+    shift = np.array([10.0, -4.0, 7.5])
+    for _ in range(50):
+        P = rng.uniform(0, 3.0, size=(3, 3))
+        ci, cj, ck = oracle.at_contrib(P[0], P[1], P[2])
+        s = np.abs([ci, cj, ck]).max()
+        assert np.abs(ci + cj + ck).max() <= 1e-13 * s
+        # rotation and translation covariance
+        Q = P @ R.T + shift
+        di, dj, dk = oracle.at_contrib(Q[0], Q[1], Q[2])
+        assert np.allclose(np.array([di, dj, dk]), np.array([ci, cj, ck]) @ R.T, rtol=0, atol=1e-12 * s)
+        # relabelling the triplet permutes the contributions
+        ei, ej, ek = oracle.at_contrib(P[1], P[2], P[0])
+        assert np.allclose([ei, ej, ek], [cj, ck, ci], rtol=0, atol=1e-12 * s)
+
+
+def test_exactly_linear_in_nu(oracle):
+    P = uniform_particles(60, 3.0, 1)
+    r1 = oracle.sweep(P, nu=1.0, mode="brute")
+    r2 = oracle.sweep(P, nu=2.0, mode="brute")
+    # scaling nu by 2 scales every rounded intermediate by 2: bit-exact
+    for a, b in zip(r1[:4], r2[:4]):
+        assert np.array_equal(2.0 * a, b)
+
+
+# --- triplet sweep and counts (Algorithm.cpp:72-139) --------------------------
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 4, 10, 57])
+def test_counts_without_cutoff_are_binomial(oracle, n):
+    P = uniform_particles(n, 3.0, n + 1) if n else np.zeros((0, 3))
+    act, poss = oracle.brute_possible(P)
+    assert act == poss == n * (n - 1) * (n - 2) // 6
+    *_, cnt = oracle.sweep(P, mode="owner")
+    assert cnt == act
+
+
+def test_reference_golden_count_1000(oracle):
+    """The reference's only particle-level known answer: C(1000,3) = 166167000
+    (tests.cpp:530-568, N = 1000 uniform in a 10^3 box)."""
+    P = uniform_particles(1000, 10.0, 926762934)
+    *_, cnt = oracle.sweep(P, mode="owner")
+    assert cnt == 166167000
+
+
+def test_brute_counts_by_hand(oracle):
+    """Four particles, cutoff 1.5: only the triangle {0,1,2} has all sides <= c."""
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [3, 0, 0]], float)
+    L = np.array([100.0, 100.0, 100.0])
+    act, poss = oracle.brute_possible(P, cutoff=1.5, L=L)
+    assert (act, poss) == (1, 4)
+    # with a small periodic box particle 3 (x = 3) sits next to particle 0 (x = 0)
+    L = np.array([3.5, 100.0, 100.0])
+    act, poss = oracle.brute_possible(P, cutoff=1.5, L=L)
+    # distances: d03 = min(3, 0.5) = 0.5, d13 = min(2, 1.5) = 1.5, d23 = |(0.5, 1)| = 1.118
+    assert act == 4
+
+
+def test_periodic_distance_known_answers(oracle):
+    L = np.array([10.0, 10.0, 10.0])
+    assert oracle.sqr_dist_periodic([0, 0, 0], [3, 4, 0], L) == 25.0
+    assert oracle.sqr_dist_periodic([0, 0, 0], [9, 0, 0], L) == 1.0        # wraps
+    assert oracle.sqr_dist_periodic([0, 0, 0], [15, 0, 0], L) == 25.0      # |d| > L: min(15, -5) = -5
+    assert oracle.sqr_dist_periodic([0, 0, 0], [21, 0, 0], L) == 121.0     # min(21, -11) = -11
+    assert oracle.sqr_dist_periodic([1, 2, 3], [1, 2, 3], L) == 0.0
+
+
+@pytest.mark.parametrize("case", ["nocut", "cut", "outside", "tiny_box"])
+def test_brute_and_owner_sweeps_bit_identical(oracle, case):
+    rng = np.random.default_rng(hash(case) % 2 ** 32)
+    n = 240
+    P = uniform_particles(n, 6.0, 17)
+    L, _ = oracle.domain_size(P)
+    cutoff = -1.0 if case == "nocut" else 2.0
+    if case == "outside":
+        # particles moved one and two box lengths away exercise the one-image rule
+        idx = rng.choice(n, 40, replace=False)
+        P[idx, rng.integers(0, 3, 40)] += L[0] * rng.choice([-2.0, -1.0, 1.0, 2.0], 40)
+    if case == "tiny_box":
+        L = np.array([4.1, 4.1, 4.1])
+    r1 = oracle.sweep(P, cutoff=cutoff, L=L, mode="brute")
+    r2 = oracle.sweep(P, cutoff=cutoff, L=L, mode="owner", threads=4)
+    assert r1[4] == r2[4]
+    for a, b in zip(r1[:4], r2[:4]):
+        assert np.array_equal(a, b)
+
+
+def test_fast_formulation_agrees_with_reference(oracle):
+    P = uniform_particles(300, 6.0, 23)
+    L, _ = oracle.domain_size(P)
+    for cutoff in (-1.0, 2.5):
+        *_, f, cnt = oracle.sweep(P, cutoff=cutoff, L=L, mode="owner")
+        ff, E, cntf = oracle.forces_fast(P, cutoff=cutoff, L=L)
+        Et, cntt = oracle.energy(P, cutoff=cutoff, L=L)
+        assert cnt == cntf == cntt
+        rel = np.linalg.norm(ff - f, axis=1) / np.linalg.norm(f, axis=1)
+        assert rel.max() < 1e-11
+        assert E == pytest.approx(Et, rel=1e-12)
+
+
+# --- integrator (structs.hpp:40-99) and domain (RegularGridDecomposition.cpp) --
+
+def test_integrator_known_answers(oracle):
+    """Two particles: no triplet, f = 0, so the closed forms hold exactly."""
+    P0 = np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.5]])
+    V0 = np.zeros((2, 3))
+    A0 = np.zeros((2, 3))
+    M = np.array([1.0, 2.0])
+    g = np.array([0.5, -9.81, 2.0])
+    dt = 0.001
+    pos, vel, acc, f, counts = oracle.simulate(P0, V0, A0, M, "p3bca", 1, dt, g, cutoff=2.0)
+    assert np.array_equal(pos, P0)                     # step 0: v = a = 0
+    assert np.allclose(vel, np.tile(g * dt / 2, (2, 1)), rtol=1e-15, atol=0)
+    assert np.array_equal(acc, np.tile(g, (2, 1)))
+    assert np.array_equal(f, np.zeros((2, 3)))
+    pos2, vel2, acc2, _, _ = oracle.simulate(P0, V0, A0, M, "auta", 2, dt, g)
+    # step 1 moves by v dt + (aX, aX, aX) dt^2 / 2: the aX-for-all-axes quirk
+    expect = P0 + (g * dt / 2) * dt + g[0] * dt * dt * 0.5
+    assert np.allclose(pos2, expect, rtol=1e-15, atol=1e-15)
+    assert np.allclose(vel2, np.tile(g * dt, (2, 1)), rtol=1e-15, atol=0)
+
+
+def test_update_uses_force_over_mass(oracle):
+    f = np.array([[2.0, -4.0, 1.0]])
+    vel, acc = oracle.update(f, np.array([2.0]), np.array([[1.0, 1.0, 1.0]]), np.zeros((1, 3)), 0.1,
+                             np.array([0.0, -9.81, 0.0]))
+    assert np.array_equal(acc, [[-1.0, -9.81 + 2.0, -0.5]])
+    assert np.array_equal(vel, [[1.0 + -1.0 * 0.05, 1.0 + (-9.81 + 2.0) * 0.05, 1.0 - 0.5 * 0.05]])
+
+
+def test_domain_size_known_answer(oracle):
+    P = np.array([[0.0, 1.0, -2.0], [10.0, 3.0, 5.0], [4.0, -1.0, 0.0]])
+    L, lo = oracle.domain_size(P)
+    assert np.array_equal(L, [abs((10 + 0.0001) - (0 - 0.0001)), abs((3 + 0.0001) - (-1 - 0.0001)),
+                              abs((5 + 0.0001) - (-2 - 0.0001))])
+    assert np.array_equal(lo, [0 - 0.0001, -1 - 0.0001, -2 - 0.0001])
+
+
+def test_simulate_is_the_manual_loop(oracle):
+    P = uniform_particles(150, 4.0, 9)
+    V = np.zeros_like(P)
+    A = np.zeros_like(P)
+    M = np.ones(150)
+    g = np.array([0.0, -9.81, 0.0])
+    dt = 0.001
+    pos, vel, acc, f, counts = oracle.simulate(P, V, A, M, "p3bca", 3, dt, g, cutoff=1.8)
+    L, _ = oracle.domain_size(P)
+    p, v, a = P.copy(), V.copy(), A.copy()
+    for s in range(3):
+        p = oracle.predictor(p, v, a, dt)
+        *_, ff, c = oracle.sweep(p, cutoff=1.8, L=L, mode="brute")
+        v, a = oracle.update(ff, M, v, a, dt, g)
+        assert c == counts[s]
+    assert np.array_equal(p, pos) and np.array_equal(v, vel) and np.array_equal(a, acc) and np.array_equal(ff, f)
+
+
+# --- the oracle's command-line driver (reference main / CSV) ------------------
+
+def test_oracle_cli_writes_reference_csv(oracle, tmp_path):
+    P = uniform_particles(20, 3.0, 4)
+    src = tmp_path / "in.csv"
+    with open(src, "w") as fh:
+        fh.write("ID, pX, pY, pZ, vX, vY, vZ, aX, aY, aZ, m\n")
+        for i in reversed(range(20)):  # IDs out of order: output must be sorted
+            fh.write(f"{i}, {P[i,0]:g}, {P[i,1]:g}, {P[i,2]:g}, 0, 0, 0, 0, 0, 0, 1\n")
+    out = tmp_path / "run.out.csv"
+    r = subprocess.run([oracle.ORACLE_CLI, "-a", "auta", "-i", "2", "-d", "0.001", "-gy", "-9.81", "-csv",
+                        str(src), "-o", str(out), "-bogus"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "CLI parser: unrecognized option: bogus" in r.stdout
+    for s in range(2):
+        path = tmp_path / f"run.out_{s}.csv"
+        lines = path.read_text().splitlines()
+        assert lines[0] == "ID, pX, pY, pZ, vX, vY, vZ, aX, aY, aZ, m, fX, fY, fZ"
+        ids = [int(l.split(",")[0]) for l in lines[1:]]
+        assert ids == list(range(20))
+        for l in lines[1:]:
+            cells = l.split(", ")
+            assert len(cells) == 14
+            for c in cells[1:]:
+                assert c == "%g" % float(c)

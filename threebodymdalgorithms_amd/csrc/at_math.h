@@ -1,0 +1,185 @@
+// at_math.h -- device-side Axilrod-Teller arithmetic for gfx950.
+//
+// Two formulations of the same potential:
+//  * the reference's gradient expression (potential/AxilrodTeller.cpp:22-93),
+//    evaluated operation for operation in IEEE FP64 (this translation unit is
+//    compiled with -ffp-contract=off, so nothing is fused), used by the
+//    bit-exact path;
+//  * the squared-distance form used by the fast paths:
+//      u = |xj-xi|^2, v = |xk-xi|^2, w = |xk-xj|^2, s = (u v w)^-1/2,
+//      A = u+v-w, B = u-v+w, C = -u+v+w, K = 3 nu / 8,
+//      E = s^3 (nu + K Q),  Q = A B C s^2,
+//      dE/du = s^5 (R v w + K (S - 2 A B)), dE/dv = s^5 (R u w + K (S - 2 A C)),
+//      dE/dw = s^5 (R u v + K (S - 2 B C)),
+//      S = BC + AC + AB, R = -3 nu / 2 - 5 K Q / 2.
+//    Fused multiply-adds are written explicitly with fma() so the rounding of
+//    this form is fixed by the source, not by the compiler.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace tb3 {
+
+// Utility::Particle::GetSqrDistPeriodic (utility/structs.hpp:106-115):
+// per axis t = |b - a|, e = min(t, L - t) with std::min's (b < a) ? b : a,
+// result ex*ex + ey*ey + ez*ez.  No wrapping of positions: for t > L the
+// second operand is negative and squares to (t - L)^2, exactly as the reference.
+__device__ __forceinline__ double sqr_dist_periodic(double ax, double ay, double az, double bx, double by,
+                                                    double bz, double Lx, double Ly, double Lz)
+{
+    const double tx = __builtin_fabs(bx - ax);
+    const double ty = __builtin_fabs(by - ay);
+    const double tz = __builtin_fabs(bz - az);
+    const double lx = Lx - tx, ly = Ly - ty, lz = Lz - tz;
+    const double ex = (lx < tx) ? lx : tx;
+    const double ey = (ly < ty) ? ly : ty;
+    const double ez = (lz < tz) ? lz : tz;
+    return ex * ex + ey * ey + ez * ez;
+}
+
+// Eigen's Vector3d::norm() as sqrt((x*x + y*y) + z*z) (AxilrodTeller.cpp:29-31).
+__device__ __forceinline__ double ref_norm3(double x, double y, double z)
+{
+    return __builtin_sqrt((x * x + y * y) + z * z);
+}
+
+struct RefPowers {
+    double d1, d2, d3, d4, d5, d6;  // AxilrodTeller.cpp:33-51
+};
+
+__device__ __forceinline__ RefPowers ref_powers(double d1)
+{
+    RefPowers p;
+    p.d1 = d1;
+    p.d2 = p.d1 * d1;
+    p.d3 = p.d2 * d1;
+    p.d4 = p.d3 * d1;
+    p.d5 = p.d4 * d1;
+    p.d6 = p.d5 * d1;
+    return p;
+}
+
+// One dVdR bracket (AxilrodTeller.cpp:68-81) for own side x and the two other
+// sides y, z in the order the source names them: dVdRa = (a; b, c),
+// dVdRb = (b; a, c), dVdRc = (c; b, a).  Left-to-right sums and products.
+__device__ __forceinline__ double ref_bracket(double nu, const RefPowers &x, const RefPowers &y,
+                                              const RefPowers &z)
+{
+    double t = -8. / (x.d4 * y.d3 * z.d3);
+    t = t - 1. / (y.d5 * z.d5);
+    t = t + 5. * y.d1 / (x.d6 * z.d5);
+    t = t + 5. * z.d1 / (x.d6 * y.d5);
+    t = t - 1. / (x.d2 * y.d3 * z.d5);
+    t = t - 1. / (x.d2 * y.d5 * z.d3);
+    t = t - 3. / (x.d4 * y.d1 * z.d5);
+    t = t - 3. / (x.d4 * y.d5 * z.d1);
+    t = t - 5. / (x.d6 * y.d1 * z.d3);
+    t = t - 5. / (x.d6 * y.d3 * z.d1);
+    t = t + 6. / (x.d4 * y.d3 * z.d3);
+    return (3. * nu / (8. * x.d1)) * t;
+}
+
+// Contribution that AxilrodTeller::CalculateForces subtracts from the force
+// of the particle in role ROLE (0: i, 1: j, 2: k) of the triplet (i, j, k),
+// AxilrodTeller.cpp:83-93.  Only the two brackets that role needs are
+// evaluated; each is the same IEEE operation sequence as in the full call, so
+// the result is bit-identical to the reference's.
+template <int ROLE>
+__device__ __forceinline__ void ref_contrib(double xi, double yi, double zi, double xj, double yj, double zj,
+                                            double xk, double yk, double zk, double nu, double &cx, double &cy,
+                                            double &cz)
+{
+    const RefPowers a = ref_powers(ref_norm3(xj - xi, yj - yi, zj - zi));  // a = rj - ri
+    const RefPowers b = ref_powers(ref_norm3(xi - xk, yi - yk, zi - zk));  // b = ri - rk
+    const RefPowers c = ref_powers(ref_norm3(xj - xk, yj - yk, zj - zk));  // c = rj - rk
+    const double dXa = xj - xi, dYa = yj - yi, dZa = zj - zi;
+    const double dXb = xk - xi, dYb = yk - yi, dZb = zk - zi;
+    const double dXc = xk - xj, dYc = yk - yj, dZc = zk - zj;
+    if constexpr (ROLE == 0) {
+        const double A = ref_bracket(nu, a, b, c);
+        const double B = ref_bracket(nu, b, a, c);
+        cx = dXa * A + dXb * B;
+        cy = dYa * A + dYb * B;
+        cz = dZa * A + dZb * B;
+    } else if constexpr (ROLE == 1) {
+        const double A = ref_bracket(nu, a, b, c);
+        const double C = ref_bracket(nu, c, b, a);
+        cx = dXa * (-A) + dXc * C;
+        cy = dYa * (-A) + dYc * C;
+        cz = dZa * (-A) + dZc * C;
+    } else {
+        const double B = ref_bracket(nu, b, a, c);
+        const double C = ref_bracket(nu, c, b, a);
+        cx = dXb * (-B) + dXc * (-C);
+        cy = dYb * (-B) + dYc * (-C);
+        cz = dZb * (-B) + dZc * (-C);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fast formulation.
+
+// 1/sqrt(x) in FP64: hardware estimate refined by two Newton steps
+// y <- y + (y/2)(1 - x y^2).
+__device__ __forceinline__ double rsqrt_f64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double r = __builtin_fma(-(x * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, r, y);
+    }
+    return y;
+}
+
+// 1/sqrt(x) in FP32: hardware estimate refined by one Newton step.
+__device__ __forceinline__ float rsqrt_f32(float x)
+{
+    float y = __builtin_amdgcn_rsqf(x);
+    const float r = __builtin_fmaf(-(x * y), y, 1.0f);
+    return __builtin_fmaf(0.5f * y, r, y);
+}
+
+template <class T>
+__device__ __forceinline__ T fma_t(T a, T b, T c);
+template <>
+__device__ __forceinline__ double fma_t<double>(double a, double b, double c)
+{
+    return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fma_t<float>(float a, float b, float c)
+{
+    return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double rsqrt_t(double x) { return rsqrt_f64(x); }
+__device__ __forceinline__ float rsqrt_t(float x) { return rsqrt_f32(x); }
+
+// Gradient coefficients of one triplet in the squared-distance form.
+// Returns eu = 2 dE/du, ev = 2 dE/dv, ew = 2 dE/dw and the energy e, so that
+//   dE/dxi = -(eu dij + ev dik), dE/dxj = eu dij - ew djk, dE/dxk = ev dik + ew djk
+// with dij = xj - xi, dik = xk - xi, djk = xk - xj.
+template <class T, bool NEED_W>
+__device__ __forceinline__ void at_coeffs(T u, T v, T w, T nu, T K, T &eu, T &ev, T &ew, T &e)
+{
+    const T uv = u * v;
+    const T s = rsqrt_t(uv * w);
+    const T s2 = s * s;
+    const T A = (u + v) - w;
+    const T B = (u - v) + w;
+    const T C = (v - u) + w;
+    const T X = B * C, Y = A * C, Z = A * B;
+    const T Q = (A * X) * s2;
+    const T R = fma_t<T>(T(-2.5) * K, Q, T(-1.5) * nu);
+    const T S = (X + Y) + Z;
+    const T KS = K * S;
+    const T m2K = T(-2) * K;
+    const T c2s5 = T(2) * (s2 * s2 * s);
+    eu = c2s5 * fma_t<T>(R, v * w, fma_t<T>(m2K, Z, KS));
+    ev = c2s5 * fma_t<T>(R, u * w, fma_t<T>(m2K, Y, KS));
+    if constexpr (NEED_W) ew = c2s5 * fma_t<T>(R, uv, fma_t<T>(m2K, X, KS));
+    else ew = T(0);
+    e = (s2 * s) * fma_t<T>(K, Q, nu);
+}
+
+}  // namespace tb3

@@ -1,0 +1,152 @@
+"""Seeded particle generators of the reference's particlegenerator tool
+(tools/*.cpp, particleToCSV.cpp:19-139), reproducing libstdc++'s random
+engines and distributions so the same seeds give the same particles.
+
+  uniform         UniformGenerator.cpp:12-28: std::mt19937(seed0),
+                  uniform_real_distribution<double>(0, 1), three draws per particle.
+  gauss           GaussGenerator.cpp:14-30: std::default_random_engine(seed0)
+                  (minstd_rand0), three normal_distribution<double>.
+  clusteredgauss  ClusteredGaussGenerator.cpp:15-47: centres from
+                  std::mt19937(seed1), offsets from default_random_engine(seed0);
+                  numParticles / numClusters per cluster.
+  grid            GridGenerator.cpp:13-29.
+
+Draw order: each particle is built with std::make_tuple(id, x-expr, y-expr,
+z-expr, ...), whose argument evaluation order C++ leaves unspecified.  GCC
+(the compiler of the reference's Linux build) evaluates them right to left, so
+the first draw lands in z, then y, then x; the same holds for the cluster
+centres.  These generators follow GCC's order.
+
+libstdc++ semantics restated:
+  * generate_canonical<double, 53>: k = ceil(53 / log2(R)) draws,
+    sum_i (g_i - min) R^i / R^k, clamped below 1 (nextafter(1, 0)).
+  * normal_distribution: Marsaglia polar method; each distribution object
+    caches its second value.
+Values are then written through the CSV text format (%g) by write_csv, as
+the tool does (particleToCSV.cpp:127-137).
+"""
+import math
+
+import numpy as np
+
+_TWO32 = 4294967296.0
+
+
+class MT19937:
+    """std::mt19937 with the standard 32-bit seeding."""
+
+    def __init__(self, seed):
+        self._bg = np.random.MT19937()
+        self._bg._legacy_seeding(int(seed) & 0xFFFFFFFF)
+
+    def raw(self, n):
+        return self._bg.random_raw(n).astype(np.uint64)
+
+
+class MinStdRand0:
+    """std::minstd_rand0 (std::default_random_engine in libstdc++): x = 16807 x mod (2^31 - 1)."""
+
+    M = 2147483647
+
+    def __init__(self, seed):
+        s = int(seed) % self.M
+        self.x = 1 if s == 0 else s
+
+    def __call__(self):
+        self.x = (self.x * 16807) % self.M
+        return self.x
+
+
+def _canonical_mt(raw_pairs):
+    # R = 2^32, k = 2: (g0 + g1 * 2^32) / 2^64
+    lo = raw_pairs[0::2].astype(np.float64)
+    hi = raw_pairs[1::2].astype(np.float64)
+    r = (lo + hi * _TWO32) / (_TWO32 * _TWO32)
+    return np.where(r >= 1.0, np.nextafter(1.0, 0.0), r)
+
+
+def _canonical_minstd(gen):
+    # min = 1, max = 2^31 - 2, R = 2^31 - 2, k = 2
+    R = 2147483646.0
+    s = float(gen() - 1)
+    s += float(gen() - 1) * R
+    r = s / (R * R)
+    return math.nextafter(1.0, 0.0) if r >= 1.0 else r
+
+
+class NormalDistribution:
+    """std::normal_distribution<double> (polar method with a cached value)."""
+
+    def __init__(self, mean, stddev):
+        self.mean, self.stddev = float(mean), float(stddev)
+        self._saved = None
+
+    def __call__(self, gen):
+        if self._saved is not None:
+            ret, self._saved = self._saved, None
+        else:
+            while True:
+                x = 2.0 * _canonical_minstd(gen) - 1.0
+                y = 2.0 * _canonical_minstd(gen) - 1.0
+                r2 = x * x + y * y
+                if not (r2 > 1.0 or r2 == 0.0):
+                    break
+            mult = math.sqrt(-2.0 * math.log(r2) / r2)
+            self._saved = x * mult
+            ret = y * mult
+        return ret * self.stddev + self.mean
+
+
+def uniform(n, box, seed0, corner=(0.0, 0.0, 0.0)):
+    """UniformGenerator: corner + U(0,1) * box per axis (draws in z, y, x order)."""
+    box = np.broadcast_to(np.asarray(box, dtype=np.float64), (3,))
+    u = _canonical_mt(MT19937(seed0).raw(6 * n)).reshape(n, 3)[:, ::-1]
+    return np.asarray(corner, dtype=np.float64) + u * box
+
+
+def gauss(n, mean, stddev, seed0, corner=(0.0, 0.0, 0.0)):
+    """GaussGenerator: corner + N(mean, stddev) per axis."""
+    mean = np.broadcast_to(np.asarray(mean, dtype=np.float64), (3,))
+    sd = np.broadcast_to(np.asarray(stddev, dtype=np.float64), (3,))
+    gen = MinStdRand0(seed0)
+    d = [NormalDistribution(mean[k], sd[k]) for k in range(3)]
+    out = np.empty((n, 3))
+    for i in range(n):
+        for k in (2, 1, 0):
+            out[i, k] = corner[k] + d[k](gen)
+    return out
+
+
+def clustered_gauss(n, nclusters, box, mean, stddev, seed0, seed1, corner=(0.0, 0.0, 0.0)):
+    """ClusteredGaussGenerator: n // nclusters particles around each of
+    nclusters uniform centres (the remainder is dropped, as the reference does)."""
+    box = np.broadcast_to(np.asarray(box, dtype=np.float64), (3,))
+    u = _canonical_mt(MT19937(seed1).raw(6 * nclusters)).reshape(nclusters, 3)[:, ::-1]
+    centres = np.asarray(corner, dtype=np.float64) + u * box
+    mean = np.broadcast_to(np.asarray(mean, dtype=np.float64), (3,))
+    sd = np.broadcast_to(np.asarray(stddev, dtype=np.float64), (3,))
+    gen = MinStdRand0(seed0)
+    d = [NormalDistribution(mean[k], sd[k]) for k in range(3)]
+    per = n // nclusters
+    out = np.empty((per * nclusters, 3))
+    r = 0
+    for c in range(nclusters):
+        for _ in range(per):
+            for k in (2, 1, 0):
+                out[r, k] = centres[c, k] + d[k](gen)
+            r += 1
+    return out
+
+
+def grid(per_dim, spacing, corner=(0.0, 0.0, 0.0)):
+    """GridGenerator: z-major, then y, then x."""
+    nx, ny, nz = per_dim
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.float64) * spacing
+    return pts + np.asarray(corner, dtype=np.float64)
+
+
+def csv_rounded(a):
+    """Values as they come back from the CSV text the tool writes (%g)."""
+    flat = np.asarray(a, dtype=np.float64).ravel()
+    return np.array([float("%g" % v) for v in flat]).reshape(np.shape(a))
