@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--cpu-baseline-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-variants", action="store_true", help="time the FP64 path only")
+    ap.add_argument("--only", choices=["fp64", "exact", "fp32"],
+                    help="time this path alone and print its figures (profiling runs; not the contract line)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -154,6 +156,11 @@ def main():
 
     P = workload()
     log(f"bench: N={N} box={BOX} cutoff={CUTOFF} steps={args.steps} warmup={args.warmup}")
+    if args.only:
+        with tb3.Engine(0) as eng:
+            r = timed_steps(eng, P, args.only, args.steps, args.warmup, sync)
+        print(json.dumps({"path": args.only, **r}), flush=True)
+        return
     with tb3.Engine(0) as eng:
         fp64 = timed_steps(eng, P, "fp64", args.steps, args.warmup, sync)
         log(f"bench: fp64 {fp64['ms_per_step']:.3f} ms/step {fp64['triplets_per_s']:.4e} triplets/s "

@@ -1,8 +1,8 @@
 // tb3_host.cpp -- host side of the C ABI declared in include/tb3.h.
 //
-// Owns the device buffers of one particle system, builds the cell grid and
-// neighbour lists for the cutoff algorithm, and dispatches the sweep kernels
-// by algorithm and precision.  No exception crosses the ABI.
+// Owns the device buffers of one particle system, keeps the cell grid of the
+// cutoff algorithm, and dispatches the sweep kernels by algorithm and
+// precision.  No exception crosses the ABI.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,7 +53,7 @@ void dev_free(T *&p)
     p = nullptr;
 }
 
-// Working set for one sweep: positions in, forces out, plus role accumulators.
+// Device arrays of one particle system (structure of arrays).
 struct Buffers {
     long long cap = 0;
     double *block = nullptr;  // 22 * cap doubles
@@ -85,6 +85,8 @@ struct Buffers {
     }
 };
 
+constexpr int MAX_LIST = 16384;  // longest neighbour list the LDS staging accepts
+
 }  // namespace
 
 struct tb3_ctx {
@@ -104,33 +106,27 @@ struct tb3_ctx {
     long long n = 0;
     std::vector<int32_t> ids;
     Buffers state;    // the particle system
-    Buffers scratch;  // device-pointer API working set
+    Buffers scratch;  // staging and the device-pointer API's working set
 
-    // cell grid and neighbour lists
+    // cell grid of the cutoff algorithm
     CellGrid grid;
     int grid_cell_cap = 0, grid_n_cap = 0;
-    int *nbr_cnt = nullptr;
-    long long *nbr_off = nullptr;
-    int *nbr = nullptr;
-    long long nbr_n_cap = 0, nbr_cap = 0;
-    int nbr_maxlen = 0;
     long long *scan_tmp = nullptr;
     long long scan_tmp_cap = 0;
+    int list_cap = 128;  // LDS neighbour-list capacity per wave (power of two)
 
     // tile triples for the all-triplets fast sweep
     unsigned long long *triples = nullptr;
     long long ntriples = 0;
     long long triples_n = -1;
 
-    // reductions and host staging
-    unsigned long long *d_count = nullptr;  // [0] sweep, [1] discard
-    double *d_energy = nullptr;
-    int *d_flags = nullptr;
-    void *h_pinned = nullptr;  // 64 bytes
+    Status *d_status = nullptr;
+    Status *h_status = nullptr;  // pinned
+    Slots *d_slots = nullptr;
 
     // phase timing (tb3_set_profiling)
-    bool prof = false;       // enabled by the caller
-    bool recording = false;  // inside a profiled step
+    bool prof = false;
+    bool recording = false;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double prof_acc[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -161,9 +157,9 @@ tb3_status guarded(tb3_ctx *c, F &&f)
     }
 }
 
+// RegularGridDecomposition::getDomainMinMax / Init (RegularGridDecomposition.cpp:24-28, 135-166)
 void domain_from_rows(long long n, const double *rows, double *L, double *lo)
 {
-    // RegularGridDecomposition::getDomainMinMax / Init (RegularGridDecomposition.cpp:24-28, 135-166)
     double mn[3], mx[3];
     for (int c = 0; c < 3; ++c) {
         mn[c] = 1.7976931348623157e308;
@@ -223,32 +219,13 @@ void configure_grid(tb3_ctx *c)
     }
 }
 
-void ensure_scan_tmp(tb3_ctx *c, long long count)
+void ensure_grid_buffers(tb3_ctx *c, int n)
 {
-    const long long need = scan_tmp_elems(count);
-    if (need > c->scan_tmp_cap) {
-        dev_alloc(c->scan_tmp, need);
-        c->scan_tmp_cap = need;
-    }
-}
-
-template <class T>
-void readback(tb3_ctx *c, hipStream_t s, T *host, const T *dev, size_t count)
-{
-    HIPCHK(hipMemcpyAsync(c->h_pinned, dev, count * sizeof(T), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::memcpy(host, c->h_pinned, count * sizeof(T));
-}
-
-// Cell list + CSR neighbour lists for the positions in d.
-void build_neighbours(tb3_ctx *c, hipStream_t s, const DeviceState &d, bool sorted)
-{
-    const int n = d.n;
     CellGrid &g = c->grid;
     if (g.ncell > c->grid_cell_cap) {
         dev_alloc(g.cell_cnt, g.ncell);
+        HIPCHK(hipMemset(g.cell_cnt, 0, (size_t)g.ncell * sizeof(int)));
         dev_alloc(g.cell_start, (long long)g.ncell + 1);
-        dev_alloc(g.cell_cur, g.ncell);
         c->grid_cell_cap = g.ncell;
     }
     if (n > c->grid_n_cap) {
@@ -256,51 +233,11 @@ void build_neighbours(tb3_ctx *c, hipStream_t s, const DeviceState &d, bool sort
         dev_alloc(g.cell_idx, n);
         c->grid_n_cap = n;
     }
-    if (n > c->nbr_n_cap) {
-        dev_alloc(c->nbr_cnt, n);
-        dev_alloc(c->nbr_off, (long long)n + 1);
-        c->nbr_n_cap = n;
+    const long long need = scan_tmp_elems(std::max<long long>(n, g.ncell));
+    if (need > c->scan_tmp_cap) {
+        dev_alloc(c->scan_tmp, need);
+        c->scan_tmp_cap = need;
     }
-    ensure_scan_tmp(c, std::max<long long>(n, g.ncell));
-    HIPCHK(hipMemsetAsync(c->d_flags, 0, 2 * sizeof(int), s));
-    const CutoffSpec cs = cutoff_spec(c);
-    launch_cells_build(s, n, d.x, d.y, d.z, g, c->d_flags, c->scan_tmp);
-    launch_nbr_count(s, n, d.x, d.y, d.z, g, cs, 0, c->nbr_cnt, c->d_flags);
-    launch_scan_i32_to_i64(s, c->nbr_cnt, c->nbr_off, n, c->scan_tmp);
-    HIPCHK(hipGetLastError());
-    long long total = 0;
-    int flags[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(c->h_pinned, c->nbr_off + n, sizeof(long long), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync((char *)c->h_pinned + 16, c->d_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::memcpy(&total, c->h_pinned, sizeof(long long));
-    std::memcpy(flags, (char *)c->h_pinned + 16, sizeof(flags));
-    int brute = 0;
-    if (flags[0]) {
-        // Non-finite positions defeat binning; the reference's brute sweep
-        // still pairs them (NaN compares false), so scan all pairs instead.
-        brute = 1;
-        HIPCHK(hipMemsetAsync(c->d_flags + 1, 0, sizeof(int), s));
-        launch_nbr_count(s, n, d.x, d.y, d.z, g, cs, 1, c->nbr_cnt, c->d_flags);
-        launch_scan_i32_to_i64(s, c->nbr_cnt, c->nbr_off, n, c->scan_tmp);
-        HIPCHK(hipMemcpyAsync(c->h_pinned, c->nbr_off + n, sizeof(long long), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync((char *)c->h_pinned + 16, c->d_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        std::memcpy(&total, c->h_pinned, sizeof(long long));
-        std::memcpy(flags, (char *)c->h_pinned + 16, sizeof(flags));
-    }
-    if (flags[1] > 65535)
-        throw Fail(TB3_EUNSUPPORTED, "a particle has " + std::to_string(flags[1]) +
-                                         " neighbours within the cutoff; at most 65535 are supported");
-    if (total > c->nbr_cap) {
-        const long long cap = std::max<long long>(total + total / 4, 1024);
-        dev_alloc(c->nbr, cap);
-        c->nbr_cap = cap;
-    }
-    c->nbr_maxlen = flags[1];
-    launch_nbr_fill(s, n, d.x, d.y, d.z, g, cs, brute, c->nbr_off, c->nbr);
-    if (sorted) launch_nbr_sort(s, n, c->nbr_off, c->nbr, c->nbr_maxlen);
-    HIPCHK(hipGetLastError());
 }
 
 void ensure_triples(tb3_ctx *c, long long n)
@@ -320,62 +257,94 @@ void ensure_triples(tb3_ctx *c, long long n)
     c->triples_n = n;
 }
 
-// One sweep over the positions in d.
-//   grad   : write dE/dx (the reference's f) into d.fx/fy/fz
-//   energy : accumulate E into c->d_energy
-// The accepted-triplet count lands in c->d_count[0].
-void sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool energy)
+void read_status(tb3_ctx *c, hipStream_t s)
 {
-    HIPCHK(hipMemsetAsync(c->d_count, 0, 2 * sizeof(unsigned long long), s));
-    HIPCHK(hipMemsetAsync(c->d_energy, 0, sizeof(double), s));
+    HIPCHK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(Status), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+int next_pow2(int v)
+{
+    int p = 64;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// The force sweep proper on positions already in d (and, for the cutoff
+// algorithm, on a cell list already built for them).
+//   grad   : dE/dx (the reference's f) into d.fx/fy/fz
+//   energy : E into the status block
+void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool energy)
+{
     const bool cut = uses_cutoff(c);
     const bool exact = c->precision == TB3_FP64_EXACT;
-    const int mixed = c->precision == TB3_FP32_MIXED ? 1 : 0;
+    const int mixed = (!exact && c->precision == TB3_FP32_MIXED) ? 1 : 0;
     const CutoffSpec cs = cutoff_spec(c);
-    if (d.n < 3) {
-        if (grad) {
-            launch_zero(s, d.fx, d.n);
-            launch_zero(s, d.fy, d.n);
-            launch_zero(s, d.fz, d.n);
-        }
-        return;
-    }
-    if (cut) {
-        build_neighbours(c, s, d, exact && grad);
-        const size_t lds = (size_t)std::max(c->nbr_maxlen, 1) * (4 * sizeof(double) + sizeof(int)) + 512;
-        if (lds > 64 * 1024)
-            throw Fail(TB3_EUNSUPPORTED, "neighbour lists of " + std::to_string(c->nbr_maxlen) +
-                                             " particles exceed the 64 KiB LDS staging of the cutoff sweep");
-    }
     const bool fast_grad = grad && !exact;
-    if (!cut && (fast_grad || energy)) {
-        ensure_triples(c, d.n);
-        if (fast_grad) {
-            launch_zero(s, d.fx, d.n);
-            launch_zero(s, d.fy, d.n);
-            launch_zero(s, d.fz, d.n);
-        }
-    }
+    if (!cut && d.n >= 3 && (fast_grad || energy)) ensure_triples(c, d.n);
     if (c->recording) HIPCHK(hipEventRecord(c->ev[2], s));
-    if (exact && grad)
-        launch_forces_exact(s, d, cut ? c->nbr_off : nullptr, cut ? c->nbr : nullptr, cs, c->nu, c->d_count);
-    if (fast_grad || energy) {
-        unsigned long long *cnt = (exact && grad) ? c->d_count + 1 : c->d_count;
-        double *e = energy ? c->d_energy : nullptr;
-        if (cut)
-            launch_forces_fast_nbr(s, d, c->nbr_off, c->nbr, c->nbr_maxlen, cs, c->nu, exact ? 0 : mixed,
-                                   fast_grad ? 1 : 0, e, cnt);
-        else
-            launch_forces_tiles(s, d, c->triples, c->ntriples, c->nu, exact ? 0 : mixed, fast_grad ? 1 : 0, e,
-                                cnt);
+    if (d.n >= 3) {
+        if (exact && grad) {
+            if (cut) launch_p3bca_exact(s, d, c->grid, cs, c->nu, c->list_cap, c->d_status);
+            else launch_auta_exact(s, d, c->nu, c->d_status);
+        }
+        if (fast_grad || energy) {
+            const int slot = (exact && grad) ? 1 : 0;  // the exact pass owns count[0]
+            if (cut) launch_p3bca_fast(s, d, c->grid, cs, c->nu, c->list_cap, mixed, fast_grad ? 1 : 0, c->d_status,
+                                       slot, energy ? 1 : 0);
+            else launch_forces_tiles(s, d, c->triples, c->ntriples, c->nu, mixed, fast_grad ? 1 : 0, c->d_status,
+                                     slot, energy ? 1 : 0);
+        }
     }
     if (c->recording) HIPCHK(hipEventRecord(c->ev[3], s));
-    if (exact && grad) launch_sum_roles(s, d);
+    if (exact && grad && d.n >= 3) launch_sum_roles(s, d);
+    launch_finalize(s, c->d_status);
     HIPCHK(hipGetLastError());
 }
 
-// Accumulates the phase times of the step just recorded (the stream has
-// been synchronised by the caller's readback).
+// Prepares a sweep over d: resets the status block, runs the predictor when
+// asked (fused into the binning pass for the cutoff algorithm), zeroes the
+// force where the sweep accumulates atomically and builds the cell list.
+void launch_prepare(tb3_ctx *c, hipStream_t s, DeviceState &d, bool predict, double dt, bool grad)
+{
+    const bool cut = uses_cutoff(c);
+    const bool exact = c->precision == TB3_FP64_EXACT;
+    const bool zero_f = grad && (!exact || d.n < 3);
+    launch_reset_status(s, c->d_status, 0);
+    if (c->recording) HIPCHK(hipEventRecord(c->ev[0], s));
+    if (cut && d.n >= 3) {
+        // the predictor is fused into the binning pass: its time is counted
+        // with the cell-list phase
+        ensure_grid_buffers(c, d.n);
+        if (c->recording) HIPCHK(hipEventRecord(c->ev[1], s));
+        launch_cells_build(s, d, predict ? 1 : 0, dt, dt * dt * 0.5, zero_f ? 1 : 0, c->grid, c->d_status,
+                           c->scan_tmp);
+    } else {
+        if (predict) launch_predict(s, d, dt, dt * dt * 0.5);
+        if (c->recording) HIPCHK(hipEventRecord(c->ev[1], s));
+        if (zero_f) {
+            launch_zero(s, d.fx, d.n);
+            launch_zero(s, d.fy, d.n);
+            launch_zero(s, d.fz, d.n);
+        }
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// Re-runs an overflowed cutoff sweep with a larger LDS list capacity.  The
+// update kernel skipped itself, so the state is just after the predictor.
+bool grow_after_overflow(tb3_ctx *c)
+{
+    if (!c->h_status->flags[2]) return false;
+    const int need = c->h_status->flags[1];
+    if (need > MAX_LIST)
+        throw Fail(TB3_EUNSUPPORTED, "a particle has " + std::to_string(need) +
+                                         " neighbours within the cutoff; at most " + std::to_string(MAX_LIST) +
+                                         " are supported");
+    c->list_cap = std::min(next_pow2(need + need / 8), MAX_LIST);
+    return true;
+}
+
 void accumulate_profile(tb3_ctx *c, unsigned long long triplets)
 {
     float ms[4] = {0, 0, 0, 0};
@@ -383,6 +352,38 @@ void accumulate_profile(tb3_ctx *c, unsigned long long triplets)
     c->prof_acc[0] += 1.0;
     for (int k = 0; k < 4; ++k) c->prof_acc[1 + k] += ms[k];
     c->prof_acc[5] += (double)triplets;
+}
+
+// One pass over d: optional predictor, sweep, optional update.  Handles the
+// neighbour-list overflow protocol and returns the accepted-triplet count.
+unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool integrate, double dt, const double *g,
+                            bool grad, bool energy, bool record)
+{
+    c->recording = record && c->prof;
+    struct Reset {
+        tb3_ctx *c;
+        ~Reset() { c->recording = false; }
+    } reset{c};
+    const bool exact = c->precision == TB3_FP64_EXACT;
+    launch_prepare(c, s, d, integrate, dt, grad);
+    launch_sweep(c, s, d, grad, energy);
+    if (integrate) launch_update(s, d, dt * 0.5, g, c->d_status);
+    if (c->recording) HIPCHK(hipEventRecord(c->ev[4], s));
+    read_status(c, s);
+    while (grow_after_overflow(c)) {
+        c->recording = false;
+        launch_reset_status(s, c->d_status, 1);  // keeps the non-finite flag of the binning pass
+        if (grad && !exact) {
+            launch_zero(s, d.fx, d.n);
+            launch_zero(s, d.fy, d.n);
+            launch_zero(s, d.fz, d.n);
+        }
+        launch_sweep(c, s, d, grad, energy);
+        if (integrate) launch_update(s, d, dt * 0.5, g, c->d_status);
+        read_status(c, s);
+    }
+    if (c->recording) accumulate_profile(c, c->h_status->count[0]);
+    return c->h_status->count[0];
 }
 
 void recompute_domain(tb3_ctx *c)
@@ -438,10 +439,15 @@ tb3_status tb3_create(int device, tb3_ctx **out)
     c->device = device;
     tb3_status st = guarded(c, [&] {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        dev_alloc(c->d_count, 2);
-        dev_alloc(c->d_energy, 1);
-        dev_alloc(c->d_flags, 2);
-        HIPCHK(hipHostMalloc(&c->h_pinned, 64, hipHostMallocDefault));
+        dev_alloc(c->d_slots, 1);
+        HIPCHK(hipMemset(c->d_slots, 0, sizeof(Slots)));
+        dev_alloc(c->d_status, 1);
+        Status init;
+        std::memset(&init, 0, sizeof(init));
+        init.slots = c->d_slots;
+        HIPCHK(hipMemcpy(c->d_status, &init, sizeof(Status), hipMemcpyHostToDevice));
+        HIPCHK(hipHostMalloc((void **)&c->h_status, sizeof(Status), hipHostMallocDefault));
+        std::memset(c->h_status, 0, sizeof(Status));
     });
     if (st != TB3_OK) {
         g_create_error = c->err;
@@ -463,19 +469,14 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.cell_of);
     dev_free(c->grid.cell_cnt);
     dev_free(c->grid.cell_start);
-    dev_free(c->grid.cell_cur);
     dev_free(c->grid.cell_idx);
-    dev_free(c->nbr_cnt);
-    dev_free(c->nbr_off);
-    dev_free(c->nbr);
     dev_free(c->scan_tmp);
     dev_free(c->triples);
-    dev_free(c->d_count);
-    dev_free(c->d_energy);
-    dev_free(c->d_flags);
+    dev_free(c->d_status);
+    dev_free(c->d_slots);
     for (hipEvent_t &e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return TB3_OK;
@@ -529,7 +530,7 @@ tb3_status tb3_set_particles(tb3_ctx *c, int64_t n, const int32_t *id, const dou
         c->state.ensure(n);
         DeviceState &d = c->state.d;
         if (n > 0) {
-            // stage n x 3 rows through facc, then split into SoA
+            // stage n x 3 rows through the role accumulators, then split into SoA
             const size_t bytes = (size_t)n * 3 * sizeof(double);
             HIPCHK(hipMemcpyAsync(d.facc, pos, bytes, hipMemcpyHostToDevice, c->stream));
             launch_soa_from_rows(c->stream, (int)n, d.facc, d.x, d.y, d.z);
@@ -587,36 +588,13 @@ static tb3_status check_ready(tb3_ctx *c)
     return TB3_OK;
 }
 
-// Predictor (optional) -> sweep -> update (optional) on the context's own
-// particles, with phase events when profiling.  Returns the accepted count.
-static unsigned long long step_once(tb3_ctx *c, bool integrate, double dt, const double *g)
-{
-    DeviceState &d = c->state.d;
-    c->recording = c->prof;
-    struct Reset {
-        tb3_ctx *c;
-        ~Reset() { c->recording = false; }
-    } reset{c};
-    if (c->recording) HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    if (integrate) launch_predict(c->stream, d, dt, dt * dt * 0.5);
-    if (c->recording) HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    sweep(c, c->stream, d, true, false);
-    if (integrate) launch_update(c->stream, d, 0, dt * 0.5, g);
-    if (c->recording) HIPCHK(hipEventRecord(c->ev[4], c->stream));
-    HIPCHK(hipGetLastError());
-    unsigned long long cnt = 0;
-    readback(c, c->stream, &cnt, c->d_count, 1);
-    if (c->recording) accumulate_profile(c, cnt);
-    return cnt;
-}
-
 tb3_status tb3_step(tb3_ctx *c, double dt, const double g[3], uint64_t *n_buffer, uint64_t *n_triplets)
 {
     if (!c || !g) return fail(c, TB3_EINVAL, "g is required");
     tb3_status st = check_ready(c);
     if (st != TB3_OK) return st;
     return guarded(c, [&] {
-        const unsigned long long cnt = step_once(c, true, dt, g);
+        const unsigned long long cnt = run_pass(c, c->stream, c->state.d, true, dt, g, true, false, true);
         if (n_buffer) *n_buffer = 1;
         if (n_triplets) *n_triplets = cnt;
     });
@@ -629,7 +607,7 @@ tb3_status tb3_run(tb3_ctx *c, int steps, double dt, const double g[3], uint64_t
     if (st != TB3_OK) return st;
     return guarded(c, [&] {
         for (int s = 0; s < steps; ++s) {
-            const unsigned long long cnt = step_once(c, true, dt, g);
+            const unsigned long long cnt = run_pass(c, c->stream, c->state.d, true, dt, g, true, false, true);
             if (counts) counts[s] = cnt;
         }
     });
@@ -641,28 +619,9 @@ tb3_status tb3_compute_forces(tb3_ctx *c, uint64_t *n_triplets)
     tb3_status st = check_ready(c);
     if (st != TB3_OK) return st;
     return guarded(c, [&] {
-        const unsigned long long cnt = step_once(c, false, 0.0, nullptr);
+        const unsigned long long cnt = run_pass(c, c->stream, c->state.d, false, 0.0, nullptr, true, false, true);
         if (n_triplets) *n_triplets = cnt;
     });
-}
-
-tb3_status tb3_set_profiling(tb3_ctx *c, int enable)
-{
-    if (!c) return TB3_EINVAL;
-    return guarded(c, [&] {
-        if (enable && !c->ev[0])
-            for (int k = 0; k < 5; ++k) HIPCHK(hipEventCreate(&c->ev[k]));
-        c->prof = enable != 0;
-        for (double &v : c->prof_acc) v = 0.0;
-    });
-}
-
-tb3_status tb3_get_profile(const tb3_ctx *cc, double *out, int64_t n)
-{
-    tb3_ctx *c = const_cast<tb3_ctx *>(cc);
-    if (!c || !out || n < 0) return fail(c, TB3_EINVAL, "bad arguments");
-    for (int64_t k = 0; k < n && k < 6; ++k) out[k] = c->prof_acc[k];
-    return TB3_OK;
 }
 
 tb3_status tb3_energy(tb3_ctx *c, double *E, uint64_t *n_triplets)
@@ -671,7 +630,7 @@ tb3_status tb3_energy(tb3_ctx *c, double *E, uint64_t *n_triplets)
     tb3_status st = check_ready(c);
     if (st != TB3_OK) return st;
     return guarded(c, [&] {
-        // the energy pass must not disturb the held force: run on scratch
+        // the energy pass must not disturb the held force: run on a copy
         c->scratch.ensure(c->n);
         DeviceState sd = c->scratch.d;
         const DeviceState &d = c->state.d;
@@ -681,10 +640,8 @@ tb3_status tb3_energy(tb3_ctx *c, double *E, uint64_t *n_triplets)
             HIPCHK(hipMemcpyAsync(sd.y, d.y, b, hipMemcpyDeviceToDevice, c->stream));
             HIPCHK(hipMemcpyAsync(sd.z, d.z, b, hipMemcpyDeviceToDevice, c->stream));
         }
-        sweep(c, c->stream, sd, false, true);
-        unsigned long long cnt = 0;
-        readback(c, c->stream, E, c->d_energy, 1);
-        readback(c, c->stream, &cnt, c->d_count, 1);
+        const unsigned long long cnt = run_pass(c, c->stream, sd, false, 0.0, nullptr, false, true, false);
+        *E = c->h_status->energy;
         if (n_triplets) *n_triplets = cnt;
     });
 }
@@ -696,11 +653,8 @@ tb3_status tb3_energy_grad(tb3_ctx *c, double *E, double *dEdx, double *dEdnu, u
     if (st != TB3_OK) return st;
     return guarded(c, [&] {
         DeviceState &d = c->state.d;
-        sweep(c, c->stream, d, true, true);
-        double e = 0.0;
-        unsigned long long cnt = 0;
-        readback(c, c->stream, &e, c->d_energy, 1);
-        readback(c, c->stream, &cnt, c->d_count, 1);
+        const unsigned long long cnt = run_pass(c, c->stream, d, false, 0.0, nullptr, true, true, false);
+        const double e = c->h_status->energy;
         if (E) *E = e;
         if (dEdnu) *dEdnu = c->nu != 0.0 ? e / c->nu : 0.0;
         if (n_triplets) *n_triplets = cnt;
@@ -727,33 +681,27 @@ static tb3_status dev_common(tb3_ctx *c, int64_t n, const double *d_pos, double 
         DeviceState sd = c->scratch.d;
         sd.n = (int)n;
         if (n > 0) launch_soa_from_rows(s, (int)n, d_pos, sd.x, sd.y, sd.z);
-        if (uses_cutoff(c) && !c->user_L && !c->have_L) {
+        if (uses_cutoff(c) && !c->have_L && n > 0) {
             std::vector<double> rows((size_t)(3 * n));
-            if (n > 0) {
-                HIPCHK(hipMemcpyAsync(rows.data(), d_pos, rows.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-                HIPCHK(hipStreamSynchronize(s));
-                domain_from_rows(n, rows.data(), c->L, c->lo);
-                c->have_L = true;
-                configure_grid(c);
-            }
+            HIPCHK(hipMemcpyAsync(rows.data(), d_pos, rows.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            domain_from_rows(n, rows.data(), c->L, c->lo);
+            c->have_L = true;
+            configure_grid(c);
         }
         const double saved_nu = c->nu;
         c->nu = nu;
         try {
-            sweep(c, s, sd, grad, true);
+            run_pass(c, s, sd, false, 0.0, nullptr, grad, true, false);
         } catch (...) {
             c->nu = saved_nu;
             throw;
         }
         c->nu = saved_nu;
-        HIPCHK(hipMemcpyAsync(d_E, c->d_energy, sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_E, &c->d_status->energy, sizeof(double), hipMemcpyDeviceToDevice, s));
         if (grad && n > 0) launch_rows_from_soa(s, (int)n, sd.fx, sd.fy, sd.fz, d_dEdx);
         HIPCHK(hipGetLastError());
-        if (n_triplets) {
-            unsigned long long cnt = 0;
-            readback(c, s, &cnt, c->d_count, 1);
-            *n_triplets = cnt;
-        }
+        if (n_triplets) *n_triplets = c->h_status->count[0];
     });
 }
 
@@ -767,6 +715,25 @@ tb3_status tb3_energy_grad_dev(tb3_ctx *c, int64_t n, const double *d_pos, doubl
                                double *d_dEdx, void *stream, uint64_t *n_triplets)
 {
     return dev_common(c, n, d_pos, nu, d_E, d_dEdx, stream, n_triplets, true);
+}
+
+tb3_status tb3_set_profiling(tb3_ctx *c, int enable)
+{
+    if (!c) return TB3_EINVAL;
+    return guarded(c, [&] {
+        if (enable && !c->ev[0])
+            for (int k = 0; k < 5; ++k) HIPCHK(hipEventCreate(&c->ev[k]));
+        c->prof = enable != 0;
+        for (double &v : c->prof_acc) v = 0.0;
+    });
+}
+
+tb3_status tb3_get_profile(const tb3_ctx *cc, double *out, int64_t n)
+{
+    tb3_ctx *c = const_cast<tb3_ctx *>(cc);
+    if (!c || !out || n < 0) return fail(c, TB3_EINVAL, "bad arguments");
+    for (int64_t k = 0; k < n && k < 6; ++k) out[k] = c->prof_acc[k];
+    return TB3_OK;
 }
 
 tb3_status tb3_selftest_fp64(tb3_ctx *c, int64_t n, const double *a, const double *b, double *sqrt_a,

@@ -18,17 +18,18 @@ struct DeviceState {
     double *facc = nullptr;  // 9n: role r, component c at facc[(3r + c) n + p]
 };
 
-// Uniform cell grid over the periodic domain, used to build neighbour lists.
+// Uniform cell grid over the periodic domain.  Cell width >= cutoff, so the
+// 27 cells around a particle's cell (fewer when a dimension has < 3 cells)
+// hold every particle within the periodic cutoff.
 struct CellGrid {
     int nc[3] = {1, 1, 1};
     int ncell = 1;
     double lo[3] = {0, 0, 0};
     double L[3] = {1, 1, 1};
     int *cell_of = nullptr;     // n
-    int *cell_cnt = nullptr;    // ncell
+    int *cell_cnt = nullptr;    // ncell (zero between steps)
     int *cell_start = nullptr;  // ncell + 1
-    int *cell_cur = nullptr;    // ncell
-    int *cell_idx = nullptr;    // n
+    int *cell_idx = nullptr;    // n, particle indices grouped by cell (order within a cell unspecified)
 };
 
 struct CutoffSpec {
@@ -37,58 +38,74 @@ struct CutoffSpec {
     double L[3] = {-1, -1, -1};
 };
 
-// Device scratch for reductions.
-struct Reduce {
-    unsigned long long *count = nullptr;  // accepted triplets
-    double *energy = nullptr;             // energy sum
-    int *flags = nullptr;                 // [0] non-finite position seen, [1] max neighbour count
+// Per-wave partial sums land in NSLOT slots one L2 line apart (slot = block
+// index mod NSLOT) instead of on one address, so thousands of waves do not
+// serialise on a single atomic; launch_finalize folds them into Status.
+constexpr int NSLOT = 256;
+constexpr int SLOT_STRIDE = 16;  // 16 x 8 bytes = one 128-byte L2 line
+struct Slots {
+    unsigned long long count[2][NSLOT * SLOT_STRIDE];
+    double energy[NSLOT * SLOT_STRIDE];
+};
+
+// Small device status block, read back with one copy per step.
+struct Status {
+    unsigned long long count[2];  // [0] accepted triplets, [1] discarded count of an auxiliary pass
+    double energy;
+    int flags[4];                 // [0] non-finite position seen, [1] longest neighbour list (recorded
+                                  // when above half the capacity), [2] list overflow
+    Slots *slots;                 // partial sums of the sweep kernels
 };
 
 // ---- launchers (all asynchronous on `s`) ----
 void launch_soa_from_rows(hipStream_t s, int n, const double *rows, double *x, double *y, double *z);
 void launch_rows_from_soa(hipStream_t s, int n, const double *x, const double *y, const double *z, double *rows);
+// Clears the status block; keep_nonfinite preserves flags[0].
+void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite);
+// Folds the slot partial sums into st->count / st->energy and clears the slots.
+void launch_finalize(hipStream_t s, Status *st);
 void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h);
-void launch_update(hipStream_t s, const DeviceState &d, int sum_roles, double hdt, const double g[3]);
+// a = g - f/m, v += a dt/2; skipped entirely when st->flags[2] (overflow) is set.
+void launch_update(hipStream_t s, const DeviceState &d, double hdt, const double g[3], const Status *st);
 void launch_sum_roles(hipStream_t s, const DeviceState &d);
 void launch_zero(hipStream_t s, double *p, long long count);
 
 // Exclusive scan of `count` int32 values into `out` (count + 1 entries, the
 // last being the total).  `tmp` must hold scan_tmp_elems(count) int64 values.
 long long scan_tmp_elems(long long count);
-void launch_scan_i32_to_i64(hipStream_t s, const int *in, long long *out, long long count, long long *tmp);
 void launch_scan_i32(hipStream_t s, const int *in, int *out, int count, long long *tmp);
 
-// Cell list: assign -> count -> scan -> scatter -> per-cell index sort.
-void launch_cells_build(hipStream_t s, int n, const double *x, const double *y, const double *z, CellGrid &g,
-                        int *flags, long long *tmp);
+// Cell list, optionally fused with the predictor and with zeroing the force:
+// bin (+ predict, + zero f) -> scan -> scatter.  Nothing downstream depends
+// on the order within a cell: the fast sweep accumulates through atomics and
+// the exact sweep sorts each owner's list by index.
+void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double dt, double h, int zero_f,
+                        CellGrid &g, Status *st, long long *tmp);
 
-// Neighbour lists (CSR): q != p with !(d2_periodic(p, q) > c2).
-// brute = 1 scans all particles (used when positions are not finite).
-void launch_nbr_count(hipStream_t s, int n, const double *x, const double *y, const double *z, const CellGrid &g,
-                      const CutoffSpec &c, int brute, int *cnt, int *flags);
-void launch_nbr_fill(hipStream_t s, int n, const double *x, const double *y, const double *z, const CellGrid &g,
-                     const CutoffSpec &c, int brute, const long long *off, int *nbr);
-// Sort each particle's list by index (wave-level bitonic sort in LDS).
-void launch_nbr_sort(hipStream_t s, int n, const long long *off, int *nbr, int maxlen);
+// Cutoff sweeps (P3BCA).  One wave per particle builds that particle's
+// neighbour list in LDS (capacity `cap`) from the cell list; a list longer
+// than cap sets st->flags[2] and records its length in st->flags[1].
+//   fast : each accepted triplet once (owner = smallest index), forces added
+//          atomically into fx/fy/fz (zeroed by launch_cells_build), energy
+//          and count into st.  mixed = FP32 per-triplet arithmetic.
+//   exact: bit-exact ordered role sums into facc; count into st.
+void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
+                       int cap, int mixed, int grad, Status *st, int count_slot, int want_energy);
+void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
+                        int cap, Status *st);
 
-// Bit-exact ordered sweep.  nbr_off == nullptr means every particle is a
-// neighbour of every other (NATA/AUTA or no cutoff).
-void launch_forces_exact(hipStream_t s, const DeviceState &d, const long long *nbr_off, const int *nbr,
-                         const CutoffSpec &c, double nu, unsigned long long *count);
-
-// Fast owner-computes sweep over neighbour lists.  Writes fx/fy/fz (if grad)
-// and adds to energy/count.  mixed = FP32 per-triplet arithmetic.
-void launch_forces_fast_nbr(hipStream_t s, const DeviceState &d, const long long *nbr_off, const int *nbr,
-                            int maxlen, const CutoffSpec &c, double nu, int mixed, int grad, double *energy,
-                            unsigned long long *count);
-
-// Fast all-triplets sweep over 64-particle tile triples t0 <= t1 <= t2.
-// Accumulates (atomically) into fx/fy/fz, which must be zeroed first.
+// All-triplet sweeps (NATA / AUTA, or P3BCA without a cutoff).
+void launch_auta_exact(hipStream_t s, const DeviceState &d, double nu, Status *st);
+// Tile triples t0 <= t1 <= t2 of 64 particles; accumulates into fx/fy/fz (zero first).
 void launch_forces_tiles(hipStream_t s, const DeviceState &d, const unsigned long long *triples,
-                         long long ntriples, double nu, int mixed, int grad, double *energy,
-                         unsigned long long *count);
+                         long long ntriples, double nu, int mixed, int grad, Status *st, int count_slot,
+                         int want_energy);
 
 // Element-wise sqrt(a) and a / b (diagnostic for the FP64 rounding).
 void launch_selftest_fp64(hipStream_t s, long long n, const double *a, const double *b, double *sq, double *q);
+
+// LDS bytes the cutoff kernels need per wave for capacity `cap`.
+size_t p3bca_fast_lds(int cap);
+size_t p3bca_exact_lds(int cap);
 
 }  // namespace tb3

@@ -13,6 +13,7 @@ namespace tb3 {
 namespace {
 
 constexpr int WAVE = 64;
+constexpr int RING = 128;  // LDS queue of accepted pairs (power of two, >= 2 * WAVE)
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -30,10 +31,32 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 
+__device__ __forceinline__ void lds_add(double *p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 inline unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
+// Per-wave partial sums go to the block's slot (see Slots).
+__device__ __forceinline__ void add_count(Status *st, int which, unsigned long long c)
+{
+    if (c) atomicAdd(&st->slots->count[which][(blockIdx.x & (NSLOT - 1)) * SLOT_STRIDE], c);
+}
+
+__device__ __forceinline__ void add_energy(Status *st, double e)
+{
+    if (e != 0.0) unsafeAtomicAdd(&st->slots->energy[(blockIdx.x & (NSLOT - 1)) * SLOT_STRIDE], e);
+}
+
+// The longest neighbour list matters only near or past the capacity.
+__device__ __forceinline__ void note_list_length(Status *st, int len, int cap)
+{
+    if (2 * len > cap) atomicMax(&st->flags[1], len);
+}
+
 // ---------------------------------------------------------------------------
-// Layout conversion and integrator.
+// Layout conversion, integrator, status.
 
 __global__ void k_soa_from_rows(int n, const double *__restrict__ rows, double *__restrict__ x,
                                 double *__restrict__ y, double *__restrict__ z)
@@ -55,40 +78,72 @@ __global__ void k_rows_from_soa(int n, const double *__restrict__ x, const doubl
     rows[3 * (long long)p + 2] = z[p];
 }
 
+__global__ void k_reset_status(Status *st, int keep_nonfinite)
+{
+    if (threadIdx.x == 0) {
+        st->count[0] = 0;
+        st->count[1] = 0;
+        st->energy = 0.0;
+        if (!keep_nonfinite) st->flags[0] = 0;
+        st->flags[1] = st->flags[2] = st->flags[3] = 0;
+    }
+}
+
+// Folds the slot partial sums into the status block and clears the slots.
+__global__ __launch_bounds__(NSLOT) void k_finalize(Status *st)
+{
+    __shared__ unsigned long long c0[NSLOT / WAVE], c1[NSLOT / WAVE];
+    __shared__ double e0[NSLOT / WAVE];
+    Slots *s = st->slots;
+    const int t = threadIdx.x, k = t * SLOT_STRIDE;
+    unsigned long long a = s->count[0][k], b = s->count[1][k];
+    double e = s->energy[k];
+    s->count[0][k] = 0;
+    s->count[1][k] = 0;
+    s->energy[k] = 0.0;
+    a = wave_sum_u64(a);
+    b = wave_sum_u64(b);
+    e = wave_sum(e);
+    if ((t & (WAVE - 1)) == 0) {
+        c0[t / WAVE] = a;
+        c1[t / WAVE] = b;
+        e0[t / WAVE] = e;
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int w = 0; w < NSLOT / WAVE; ++w) {
+            st->count[0] += c0[w];
+            st->count[1] += c1[w];
+            st->energy += e0[w];
+        }
+    }
+}
+
 // Particle::UpdatePredictorStage (utility/structs.hpp:40-55):
 // p = (p + v*dt) + aX*h with h = (dt*dt)*0.5, aX for all three axes.
-__global__ void k_predict(DeviceState d, double dt, double h)
+__device__ __forceinline__ void predict_one(const DeviceState &d, int p, double dt, double h)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= d.n) return;
     const double aX = d.ax[p];
     d.x[p] = d.x[p] + d.vx[p] * dt + aX * h;
     d.y[p] = d.y[p] + d.vy[p] * dt + aX * h;
     d.z[p] = d.z[p] + d.vz[p] * dt + aX * h;
 }
 
-// Algorithm::SumUpParticles (Algorithm.cpp:245-249) then Particle::Update
-// (structs.hpp:57-99): a = g - f/m, v = v + a*(dt*0.5).
-__global__ void k_update(DeviceState d, int sum_roles, double hdt, double gx, double gy, double gz)
+__global__ void k_predict(DeviceState d, double dt, double h)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.n) return;
-    double f[3];
-    if (sum_roles) {
-        const long long n = d.n;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            f[c] = d.facc[(0 + c) * n + p] + (d.facc[(3 + c) * n + p] + d.facc[(6 + c) * n + p]);
-        d.fx[p] = f[0];
-        d.fy[p] = f[1];
-        d.fz[p] = f[2];
-    } else {
-        f[0] = d.fx[p];
-        f[1] = d.fy[p];
-        f[2] = d.fz[p];
-    }
+    predict_one(d, p, dt, h);
+}
+
+// Particle::Update (structs.hpp:57-99): a = g - f/m, v = v + a*(dt*0.5).
+__global__ void k_update(DeviceState d, double hdt, double gx, double gy, double gz, const Status *st)
+{
+    if (st && st->flags[2]) return;  // the sweep overflowed: the host re-runs it first
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.n) return;
     const double m = d.m[p];
-    const double a0 = gx - f[0] / m, a1 = gy - f[1] / m, a2 = gz - f[2] / m;
+    const double a0 = gx - d.fx[p] / m, a1 = gy - d.fy[p] / m, a2 = gz - d.fz[p] / m;
     d.vx[p] = d.vx[p] + a0 * hdt;
     d.vy[p] = d.vy[p] + a1 * hdt;
     d.vz[p] = d.vz[p] + a2 * hdt;
@@ -97,6 +152,7 @@ __global__ void k_update(DeviceState d, int sum_roles, double hdt, double gx, do
     d.az[p] = a2;
 }
 
+// Algorithm::SumUpParticles (Algorithm.cpp:245-249): f = b0 + (b1 + b2).
 __global__ void k_sum_roles(DeviceState d)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,12 +167,6 @@ __global__ void k_zero(double *p, long long count)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = 0.0;
-}
-
-__global__ void k_zero_i32(int *p, long long count)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -201,176 +251,346 @@ __device__ __forceinline__ int cell_coord(double x, double lo, double L, int nc)
     if (nc == 1) return 0;
     double t = (x - lo) / L;
     t = t - __builtin_floor(t);
-    int i = (int)(t * (double)nc);
+    const int i = (int)(t * (double)nc);
     return i < 0 ? 0 : (i >= nc ? nc - 1 : i);
 }
 
-__global__ void k_cell_assign(int n, const double *__restrict__ x, const double *__restrict__ y,
-                              const double *__restrict__ z, GridDev g, int *__restrict__ cell_of,
-                              int *__restrict__ cell_cnt, int *__restrict__ flags)
+__device__ __forceinline__ bool finite3(double x, double y, double z)
+{
+    return __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
+}
+
+// Wrapped cell of a position; non-finite positions go to cell 0.
+__device__ __forceinline__ void cell_of_pos(const GridDev &g, double x, double y, double z, int &cx, int &cy, int &cz)
+{
+    if (finite3(x, y, z)) {
+        cx = cell_coord(x, g.lo[0], g.L[0], g.nc[0]);
+        cy = cell_coord(y, g.lo[1], g.L[1], g.nc[1]);
+        cz = cell_coord(z, g.lo[2], g.L[2], g.nc[2]);
+    } else {
+        cx = cy = cz = 0;
+    }
+}
+
+__global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_f, GridDev g,
+                      int *__restrict__ cell_of, int *__restrict__ cell_cnt, Status *st)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const double px = x[p], py = y[p], pz = z[p];
-    int c = 0;
-    if (__builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(pz)) {
-        const int ix = cell_coord(px, g.lo[0], g.L[0], g.nc[0]);
-        const int iy = cell_coord(py, g.lo[1], g.L[1], g.nc[1]);
-        const int iz = cell_coord(pz, g.lo[2], g.L[2], g.nc[2]);
-        c = (iz * g.nc[1] + iy) * g.nc[0] + ix;
-    } else {
-        atomicOr(&flags[0], 1);
+    if (p >= d.n) return;
+    if (predict) predict_one(d, p, dt, h);
+    if (zero_f) {
+        d.fx[p] = 0.0;
+        d.fy[p] = 0.0;
+        d.fz[p] = 0.0;
     }
+    const double px = d.x[p], py = d.y[p], pz = d.z[p];
+    if (!finite3(px, py, pz)) atomicOr(&st->flags[0], 1);
+    int cx, cy, cz;
+    cell_of_pos(g, px, py, pz, cx, cy, cz);
+    const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
     cell_of[p] = c;
     atomicAdd(&cell_cnt[c], 1);
 }
 
+// Places each particle in its cell's range, counting the cell's counter back
+// down to zero so the counters are clear for the next step.
 __global__ void k_cell_scatter(int n, const int *__restrict__ cell_of, const int *__restrict__ cell_start,
-                               int *__restrict__ cell_cur, int *__restrict__ cell_idx)
+                               int *__restrict__ cell_cnt, int *__restrict__ cell_idx)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int c = cell_of[p];
-    const int slot = atomicAdd(&cell_cur[c], 1);
+    const int slot = atomicSub(&cell_cnt[c], 1) - 1;
     cell_idx[cell_start[c] + slot] = p;
 }
 
-// Sort each cell's particle indices ascending so the cell order, and every
-// list built from it, is independent of atomic scheduling.
-__global__ void k_cell_sort(int ncell, const int *__restrict__ cell_start, int *__restrict__ cell_idx)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const int b = cell_start[c], e = cell_start[c + 1];
-    for (int i = b + 1; i < e; ++i) {
-        const int v = cell_idx[i];
-        int j = i - 1;
-        while (j >= b && cell_idx[j] > v) {
-            cell_idx[j + 1] = cell_idx[j];
-            --j;
-        }
-        cell_idx[j + 1] = v;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Neighbour lists.
+// Cutoff sweeps: shared parameters and the in-wave neighbour search.
 
-struct NbrDev {
+struct CutDev {
     int n;
     const double *x, *y, *z;
     GridDev g;
-    const int *cell_of, *cell_start, *cell_idx;
-    int use_cut;
+    const int *cell_start, *cell_idx;
     double c2, Lx, Ly, Lz;
+    double nu;
+    int cap;
+    double *fx, *fy, *fz;
+    double *facc;
+    Status *st;
+    int count_slot;
+    int want_energy;
 };
 
-__device__ __forceinline__ bool is_neighbour(const NbrDev &a, double px, double py, double pz, int q)
-{
-    if (!a.use_cut) return true;
-    const double d2 = sqr_dist_periodic(px, py, pz, a.x[q], a.y[q], a.z[q], a.Lx, a.Ly, a.Lz);
-    return !(d2 > a.c2);
-}
+// The candidate set of owner p: the particles of the 27 (or fewer) cells
+// around p's cell, or every particle when some position is not finite (the
+// reference's brute sweep pairs NaN positions with everything).  Lanes
+// 0..ncells-1 publish one cell's (start, exclusive prefix) in LDS; the
+// flattened candidate t maps back to a particle with a short search.
+struct Candidates {
+    int total;
+    int ncells;
+    bool brute;
+};
 
-// Visits every q != p in the 27 (or fewer) cells around p's cell.
-template <class F>
-__device__ __forceinline__ void for_each_candidate(const NbrDev &a, int p, F &&f)
+__device__ __forceinline__ Candidates gather_candidates(const CutDev &a, double px, double py, double pz,
+                                                        int *sh_start, int *sh_pre)
 {
-    const int c = a.cell_of[p];
-    const int ncx = a.g.nc[0], ncy = a.g.nc[1], ncz = a.g.nc[2];
-    const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-    const int rx = ncx >= 3 ? 1 : 0, ry = ncy >= 3 ? 1 : 0, rz = ncz >= 3 ? 1 : 0;
-    for (int dz = -rz; dz <= rz; ++dz) {
-        int zz = cz + dz;
-        zz = zz < 0 ? zz + ncz : (zz >= ncz ? zz - ncz : zz);
-        for (int dy = -ry; dy <= ry; ++dy) {
-            int yy = cy + dy;
-            yy = yy < 0 ? yy + ncy : (yy >= ncy ? yy - ncy : yy);
-            for (int dx = -rx; dx <= rx; ++dx) {
-                int xx = cx + dx;
-                xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
-                const int cc = (zz * ncy + yy) * ncx + xx;
-                const int e = a.cell_start[cc + 1];
-                for (int t = a.cell_start[cc]; t < e; ++t) {
-                    const int q = a.cell_idx[t];
-                    if (q != p) f(q);
-                }
-            }
-        }
+    Candidates c;
+    c.brute = a.st->flags[0] != 0;
+    if (c.brute) {
+        c.total = a.n;
+        c.ncells = 0;
+        return c;
     }
-}
-
-__global__ void k_nbr_count(NbrDev a, int brute, int *__restrict__ cnt, int *__restrict__ flags)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
-    const double px = a.x[p], py = a.y[p], pz = a.z[p];
-    int k = 0;
-    if (brute) {
-        for (int q = 0; q < a.n; ++q)
-            if (q != p && is_neighbour(a, px, py, pz, q)) ++k;
-    } else {
-        for_each_candidate(a, p, [&](int q) {
-            if (is_neighbour(a, px, py, pz, q)) ++k;
-        });
-    }
-    cnt[p] = k;
-    atomicMax(&flags[1], k);
-}
-
-__global__ void k_nbr_fill(NbrDev a, int brute, const long long *__restrict__ off, int *__restrict__ nbr)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
-    const double px = a.x[p], py = a.y[p], pz = a.z[p];
-    long long w = off[p];
-    if (brute) {
-        for (int q = 0; q < a.n; ++q)
-            if (q != p && is_neighbour(a, px, py, pz, q)) nbr[w++] = q;
-    } else {
-        for_each_candidate(a, p, [&](int q) {
-            if (is_neighbour(a, px, py, pz, q)) nbr[w++] = q;
-        });
-    }
-}
-
-// One wave per particle: bitonic sort of its list in LDS.
-__global__ __launch_bounds__(WAVE) void k_nbr_sort(int n, const long long *__restrict__ off, int *__restrict__ nbr)
-{
-    extern __shared__ int sh_sort[];
-    const int p = blockIdx.x;
-    const long long base = off[p];
-    const int len = (int)(off[p + 1] - base);
-    if (len <= 1) return;
     const int lane = lane_id();
-    int P = WAVE;
-    while (P < len) P <<= 1;
-    for (int t = lane; t < P; t += WAVE) sh_sort[t] = t < len ? nbr[base + t] : INT_MAX;
+    int cx, cy, cz;
+    cell_of_pos(a.g, px, py, pz, cx, cy, cz);
+    const int ncx = a.g.nc[0], ncy = a.g.nc[1], ncz = a.g.nc[2];
+    const int wx = ncx >= 3 ? 3 : 1, wy = ncy >= 3 ? 3 : 1, wz = ncz >= 3 ? 3 : 1;
+    c.ncells = wx * wy * wz;
+    int cnt = 0, st = 0;
+    if (lane < c.ncells) {
+        const int ox = lane % wx, oy = (lane / wx) % wy, oz = lane / (wx * wy);
+        int xx = cx + (wx == 3 ? ox - 1 : 0), yy = cy + (wy == 3 ? oy - 1 : 0), zz = cz + (wz == 3 ? oz - 1 : 0);
+        xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
+        yy = yy < 0 ? yy + ncy : (yy >= ncy ? yy - ncy : yy);
+        zz = zz < 0 ? zz + ncz : (zz >= ncz ? zz - ncz : zz);
+        const int cc = (zz * ncy + yy) * ncx + xx;
+        st = a.cell_start[cc];
+        cnt = a.cell_start[cc + 1] - st;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane < c.ncells) {
+        sh_start[lane] = st;
+        sh_pre[lane] = incl - cnt;
+    }
+    c.total = __shfl(incl, WAVE - 1);
     __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = lane; t < P; t += WAVE) {
-                const int u = t ^ j;
-                if (u > t) {
-                    const int va = sh_sort[t], vb = sh_sort[u];
-                    const bool up = (t & k) == 0;
-                    if ((va > vb) == up) {
-                        sh_sort[t] = vb;
-                        sh_sort[u] = va;
-                    }
-                }
+    return c;
+}
+
+__device__ __forceinline__ int candidate(const CutDev &a, const Candidates &c, const int *sh_start,
+                                         const int *sh_pre, int t)
+{
+    if (c.brute) return t;
+    int lo = 0, hi = c.ncells - 1;  // last cell k with sh_pre[k] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sh_pre[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return a.cell_idx[sh_start[lo] + (t - sh_pre[lo])];
+}
+
+__device__ __forceinline__ bool within_cutoff(const CutDev &a, double ax, double ay, double az, double bx, double by,
+                                              double bz)
+{
+    // negation of the rejection test of Algorithm.cpp:95-97
+    return !(sqr_dist_periodic(ax, ay, az, bx, by, bz, a.Lx, a.Ly, a.Lz) > a.c2);
+}
+
+// Owner slot for block b: blocks are spread round-robin over the 8 XCDs, so
+// block b runs on XCD b % 8; giving each XCD a contiguous stretch of the
+// cell-ordered particles keeps a cell neighbourhood in one XCD's L2.
+__device__ __forceinline__ int owner_slot(int b, int n)
+{
+    const int chunk = (n + 7) >> 3;
+    return (b & 7) * chunk + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Fast cutoff sweep: every accepted triplet once.
+//
+// Wave per owner p.  The owner collects N+(p), its neighbours with a larger
+// index, in LDS, walks the unordered pairs {a, b} of N+(p) in a folded order
+// (row d pairs a with (a + d) mod len, so all 64 lanes test distinct pairs),
+// applies the periodic cutoff test to the third side, queues accepted pairs
+// and evaluates 64 queued triplets at a time.  The owner's gradient stays in
+// registers; the two partners' gradients go to per-neighbour LDS
+// accumulators (LDS atomics), flushed to global memory with one atomic per
+// neighbour and component at the end.
+
+template <class T, bool GRAD>
+__global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
+{
+    extern __shared__ double smem[];
+    const int cap = a.cap;
+    double *sx = smem;
+    double *sy = sx + cap;
+    double *sz = sy + cap;
+    double *gx_ = sz + cap;
+    double *gy_ = gx_ + cap;
+    double *gz_ = gy_ + cap;
+    int *sid = reinterpret_cast<int *>(gz_ + cap);
+    unsigned *ring = reinterpret_cast<unsigned *>(sid + cap);
+    int *sh_start = reinterpret_cast<int *>(ring + RING);
+    int *sh_pre = sh_start + 32;
+
+    const int slot = owner_slot(blockIdx.x, a.n);
+    if (slot >= a.n) return;
+    const int p = a.cell_idx[slot];
+    const int lane = lane_id();
+    const double xp = a.x[p], yp = a.y[p], zp = a.z[p];
+
+    // 1. neighbour list N+(p)
+    const Candidates cand = gather_candidates(a, xp, yp, zp, sh_start, sh_pre);
+    int len = 0;
+    for (int t0 = 0; t0 < cand.total; t0 += WAVE) {
+        const int t = t0 + lane;
+        bool ok = t < cand.total;
+        int q = 0;
+        double qx = 0, qy = 0, qz = 0;
+        if (ok) {
+            q = candidate(a, cand, sh_start, sh_pre, t);
+            ok = q > p;
+            if (ok) {
+                qx = a.x[q];
+                qy = a.y[q];
+                qz = a.z[q];
+                ok = within_cutoff(a, xp, yp, zp, qx, qy, qz);
             }
-            __syncthreads();
+        }
+        const unsigned long long m = __ballot(ok);
+        const int pos = len + __popcll(m & __lanemask_lt());
+        if (ok && pos < cap) {
+            sx[pos] = qx;
+            sy[pos] = qy;
+            sz[pos] = qz;
+            gx_[pos] = 0.0;
+            gy_[pos] = 0.0;
+            gz_[pos] = 0.0;
+            sid[pos] = q;
+        }
+        len += __popcll(m);
+    }
+    if (lane == 0) note_list_length(a.st, len, cap);
+    if (len > cap) {
+        if (lane == 0) atomicOr(&a.st->flags[2], 1);
+        return;
+    }
+    __syncthreads();
+
+    // 2. pairs of N+(p)
+    const T nu = (T)a.nu, K = (T)(0.375 * a.nu);
+    double gpx = 0.0, gpy = 0.0, gpz = 0.0, esum = 0.0;
+    unsigned cnt = 0;
+    int head = 0, qn = 0;
+
+    auto flush = [&](int nq) {
+        __syncthreads();
+        if (lane < nq) {
+            const unsigned e = ring[(head + lane) & (RING - 1)];
+            const int ia = (int)(e & 0xffffu), ib = (int)(e >> 16);
+            const double xj = sx[ia], yj = sy[ia], zj = sz[ia];
+            const double xk = sx[ib], yk = sy[ib], zk = sz[ib];
+            const double djx = xj - xp, djy = yj - yp, djz = zj - zp;
+            const double dkx = xk - xp, dky = yk - yp, dkz = zk - zp;
+            const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
+            T u, v, w;
+            if constexpr (sizeof(T) == 8) {
+                u = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                v = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                w = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+            } else {
+                u = (T)__builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                v = (T)__builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                w = (T)__builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+            }
+            T eu, ev, ew, et;
+            at_coeffs<T, GRAD>(u, v, w, nu, K, eu, ev, ew, et);
+            if (GRAD) {
+                const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
+                gpx = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gpx));
+                gpy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, gpy));
+                gpz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, gpz));
+                lds_add(&gx_[ia], __builtin_fma(cu, djx, -cw * ex));
+                lds_add(&gy_[ia], __builtin_fma(cu, djy, -cw * ey));
+                lds_add(&gz_[ia], __builtin_fma(cu, djz, -cw * ez));
+                lds_add(&gx_[ib], __builtin_fma(cv, dkx, cw * ex));
+                lds_add(&gy_[ib], __builtin_fma(cv, dky, cw * ey));
+                lds_add(&gz_[ib], __builtin_fma(cv, dkz, cw * ez));
+            }
+            esum += (double)et;
+            ++cnt;
+        }
+        __syncthreads();
+        head = (head + nq) & (RING - 1);
+        qn -= nq;
+    };
+
+    const int H = (len - 1) / 2;
+    const int full = H * len;
+    const int total = len * (len - 1) / 2;
+    const float inv_len = len > 0 ? 1.0f / (float)len : 0.0f;
+    for (int t0 = 0; t0 < total; t0 += WAVE) {
+        const int t = t0 + lane;
+        bool ok = t < total;
+        int ia = 0, ib = 0;
+        if (ok) {
+            if (t < full) {
+                // t = d * len + r with 0 <= r < len: float estimate, exact fix-up
+                int d = (int)((float)t * inv_len);
+                int r = t - d * len;
+                while (r < 0) {
+                    --d;
+                    r += len;
+                }
+                while (r >= len) {
+                    ++d;
+                    r -= len;
+                }
+                ia = r;
+                ib = ia + d + 1;
+                if (ib >= len) ib -= len;
+            } else {
+                ia = t - full;
+                ib = ia + len / 2;
+            }
+            ok = within_cutoff(a, sx[ia], sy[ia], sz[ia], sx[ib], sy[ib], sz[ib]);
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok) ring[(head + qn + __popcll(m & __lanemask_lt())) & (RING - 1)] = (unsigned)ia | ((unsigned)ib << 16);
+        qn += __popcll(m);
+        if (qn >= WAVE) flush(WAVE);
+    }
+    if (qn > 0) flush(qn);
+
+    // 3. write back
+    if (GRAD) {
+        for (int t = lane; t < len; t += WAVE) {
+            const int q = sid[t];
+            unsafeAtomicAdd(&a.fx[q], gx_[t]);
+            unsafeAtomicAdd(&a.fy[q], gy_[t]);
+            unsafeAtomicAdd(&a.fz[q], gz_[t]);
+        }
+        gpx = wave_sum(gpx);
+        gpy = wave_sum(gpy);
+        gpz = wave_sum(gpz);
+        if (lane == 0) {
+            unsafeAtomicAdd(&a.fx[p], gpx);
+            unsafeAtomicAdd(&a.fy[p], gpy);
+            unsafeAtomicAdd(&a.fz[p], gpz);
         }
     }
-    for (int t = lane; t < len; t += WAVE) nbr[base + t] = sh_sort[t];
+    if (a.want_energy) esum = wave_sum(esum);
+    const unsigned long long c = wave_sum_u64(cnt);
+    if (lane == 0) {
+        if (a.want_energy) add_energy(a.st, esum);
+        add_count(a.st, a.count_slot, c);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Bit-exact ordered sweep.
 //
-// One wave owns one (particle p, role r) pair and reproduces the sequential
-// accumulation the reference performs into b_r[p].f (Algorithm.cpp:72-139):
+// Reproduces the sequential accumulation the reference performs into
+// b_r[p].f (Algorithm.cpp:72-139) for owner p and role r:
 //   role 0 (p = i): pairs (j, k), p < j < k, lexicographic;
 //   role 1 (p = j): pairs (i, k), i < p < k, lexicographic;
 //   role 2 (p = k): pairs (i, j), i < j < p, lexicographic;
@@ -379,49 +599,32 @@ __global__ __launch_bounds__(WAVE) void k_nbr_sort(int n, const long long *__res
 // to an LDS ring, 64 queued triplets are evaluated in parallel with the
 // reference's expression, and three lanes fold the contributions into the
 // x, y, z accumulators strictly in stream order: acc = acc - c.
+// The owner's neighbour list is index-sorted: explicit (LDS) for the cutoff
+// algorithm, implicit (every other particle) otherwise.
 
-struct ExactDev {
-    int n;
-    const double *x, *y, *z;
-    const long long *nbr_off;
-    const int *nbr;
-    int use_cut;
-    double c2, Lx, Ly, Lz;
-    double nu;
-    double *facc;
-    unsigned long long *count;
+struct ExactShared {
+    int qa[RING], qb[RING];
+    double cb[WAVE * 3];
 };
 
-constexpr int RING = 128;
+struct ImplicitList {
+    int p;
+    __device__ int operator()(int t) const { return t < p ? t : t + 1; }
+};
 
-template <int ROLE>
-__device__ void exact_owner(const ExactDev &a, int p)
+struct LdsList {
+    const int *sid;
+    __device__ int operator()(int t) const { return sid[t]; }
+};
+
+template <int ROLE, class List>
+__device__ void exact_role(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+                           int p, const List &elem, int len, int split, bool use_cut, double c2, double Lx,
+                           double Ly, double Lz, double nu, ExactShared &sh, double &acc,
+                           unsigned long long &accepted)
 {
-    __shared__ int qa[RING], qb[RING];
-    __shared__ double cb[WAVE * 3];
     const int lane = lane_id();
-    const int n = a.n;
-    const double xp = a.x[p], yp = a.y[p], zp = a.z[p];
-
-    // The owner's neighbour list, sorted by index: explicit (CSR) or implicit
-    // (every other particle).  split = number of entries below p.
-    long long base = 0;
-    int len, split;
-    const bool implicit = a.nbr_off == nullptr;
-    if (implicit) {
-        len = n - 1;
-        split = p;
-    } else {
-        base = a.nbr_off[p];
-        len = (int)(a.nbr_off[p + 1] - base);
-        int below = 0;
-        for (int t = lane; t < len; t += WAVE) below += a.nbr[base + t] < p ? 1 : 0;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o);
-        split = below;
-    }
-    auto elem = [&](int t) -> int { return implicit ? (t < p ? t : t + 1) : a.nbr[base + t]; };
-
+    const double xp = x[p], yp = y[p], zp = z[p];
     int a0, a1, b1;
     bool tri;
     if constexpr (ROLE == 0) {
@@ -431,32 +634,30 @@ __device__ void exact_owner(const ExactDev &a, int p)
     } else {
         a0 = 0; a1 = split; b1 = split; tri = true;
     }
-
-    double acc = 0.0;  // lanes 0, 1, 2 hold x, y, z
+    acc = 0.0;  // lanes 0, 1, 2 hold x, y, z
     int head = 0, qn = 0;
-    unsigned long long accepted = 0;
 
     auto flush = [&](int cnt) {
         __syncthreads();
         if (lane < cnt) {
-            const int slot = (head + lane) & (RING - 1);
-            const int ea = qa[slot], eb = qb[slot];
-            const double xa = a.x[ea], ya = a.y[ea], za = a.z[ea];
-            const double xb = a.x[eb], yb = a.y[eb], zb = a.z[eb];
+            const int s = (head + lane) & (RING - 1);
+            const int ea = sh.qa[s], eb = sh.qb[s];
+            const double xa = x[ea], ya = y[ea], za = z[ea];
+            const double xb = x[eb], yb = y[eb], zb = z[eb];
             double cx, cy, cz;
             if constexpr (ROLE == 0)
-                ref_contrib<0>(xp, yp, zp, xa, ya, za, xb, yb, zb, a.nu, cx, cy, cz);
+                ref_contrib<0>(xp, yp, zp, xa, ya, za, xb, yb, zb, nu, cx, cy, cz);
             else if constexpr (ROLE == 1)
-                ref_contrib<1>(xa, ya, za, xp, yp, zp, xb, yb, zb, a.nu, cx, cy, cz);
+                ref_contrib<1>(xa, ya, za, xp, yp, zp, xb, yb, zb, nu, cx, cy, cz);
             else
-                ref_contrib<2>(xa, ya, za, xb, yb, zb, xp, yp, zp, a.nu, cx, cy, cz);
-            cb[lane * 3 + 0] = cx;
-            cb[lane * 3 + 1] = cy;
-            cb[lane * 3 + 2] = cz;
+                ref_contrib<2>(xa, ya, za, xb, yb, zb, xp, yp, zp, nu, cx, cy, cz);
+            sh.cb[lane * 3 + 0] = cx;
+            sh.cb[lane * 3 + 1] = cy;
+            sh.cb[lane * 3 + 2] = cz;
         }
         __syncthreads();
         if (lane < 3) {
-            for (int t = 0; t < cnt; ++t) acc = acc - cb[t * 3 + lane];
+            for (int t = 0; t < cnt; ++t) acc = acc - sh.cb[t * 3 + lane];
         }
         __syncthreads();
         head = (head + cnt) & (RING - 1);
@@ -465,7 +666,7 @@ __device__ void exact_owner(const ExactDev &a, int p)
 
     for (int ia = a0; ia < a1; ++ia) {
         const int ea = elem(ia);
-        const double xa = a.x[ea], ya = a.y[ea], za = a.z[ea];
+        const double xa = x[ea], ya = y[ea], za = z[ea];
         const int bstart = tri ? ia + 1 : split;
         for (int b = bstart; b < b1; b += WAVE) {
             const int bl = b + lane;
@@ -473,16 +674,13 @@ __device__ void exact_owner(const ExactDev &a, int p)
             int eb = 0;
             if (ok) {
                 eb = elem(bl);
-                if (a.use_cut) {
-                    const double d2 = sqr_dist_periodic(xa, ya, za, a.x[eb], a.y[eb], a.z[eb], a.Lx, a.Ly, a.Lz);
-                    ok = !(d2 > a.c2);
-                }
+                if (use_cut) ok = !(sqr_dist_periodic(xa, ya, za, x[eb], y[eb], z[eb], Lx, Ly, Lz) > c2);
             }
             const unsigned long long m = __ballot(ok);
             if (ok) {
-                const int slot = (head + qn + __popcll(m & __lanemask_lt())) & (RING - 1);
-                qa[slot] = ea;
-                qb[slot] = eb;
+                const int s = (head + qn + __popcll(m & __lanemask_lt())) & (RING - 1);
+                sh.qa[s] = ea;
+                sh.qb[s] = eb;
             }
             const int got = __popcll(m);
             qn += got;
@@ -491,176 +689,113 @@ __device__ void exact_owner(const ExactDev &a, int p)
         }
     }
     if (qn > 0) flush(qn);
-
-    if (lane < 3) a.facc[(long long)(3 * ROLE + lane) * n + p] = acc;
-    if (ROLE == 0 && lane == 0 && accepted) atomicAdd(a.count, accepted);
 }
 
-// Wave w handles role w % 3 of particle (w / 3) (role 2 in reverse so the
-// longest streams of every role start first).
-__global__ __launch_bounds__(WAVE) void k_forces_exact(ExactDev a)
+// NATA / AUTA (every triplet): wave w handles role w % 3 of particle w / 3
+// (role 2 in reverse so the longest streams of every role start first).
+struct AutaExactDev {
+    int n;
+    const double *x, *y, *z;
+    double nu;
+    double *facc;
+    Status *st;
+};
+
+__global__ __launch_bounds__(WAVE) void k_auta_exact(AutaExactDev a)
 {
+    __shared__ ExactShared sh;
     const int w = blockIdx.x;
     const int role = w % 3, q = w / 3;
     if (q >= a.n) return;
-    if (role == 0) exact_owner<0>(a, q);
-    else if (role == 1) exact_owner<1>(a, q);
-    else exact_owner<2>(a, a.n - 1 - q);
+    const int lane = lane_id();
+    const int p = role == 2 ? a.n - 1 - q : q;
+    const ImplicitList list{p};
+    double acc = 0.0;
+    unsigned long long accepted = 0;
+    if (role == 0)
+        exact_role<0>(a.x, a.y, a.z, p, list, a.n - 1, p, false, 0, 0, 0, 0, a.nu, sh, acc, accepted);
+    else if (role == 1)
+        exact_role<1>(a.x, a.y, a.z, p, list, a.n - 1, p, false, 0, 0, 0, 0, a.nu, sh, acc, accepted);
+    else
+        exact_role<2>(a.x, a.y, a.z, p, list, a.n - 1, p, false, 0, 0, 0, 0, a.nu, sh, acc, accepted);
+    if (lane < 3) a.facc[(long long)(3 * role + lane) * a.n + p] = acc;
+    if (role == 0 && lane == 0) add_count(a.st, 0, accepted);
 }
 
-// ---------------------------------------------------------------------------
-// Fast owner-computes sweep over neighbour lists (cutoff algorithm).
-//
-// One wave per owner p.  Its neighbours are staged in LDS; the wave walks the
-// unordered neighbour pairs {a, b} in a folded order (row d pairs a with
-// (a + d) mod len) so every lane tests a distinct pair, applies the reference's
-// periodic cutoff test to the third side, appends accepted pairs to an LDS
-// ring and evaluates 64 queued triplets at a time.  Each triplet contributes
-// only dE/dx_p here; the triplet's other two particles compute their own
-// share in their own waves.  Energy and the unique-triplet count are taken
-// by the owner with the smallest index.
-
-struct FastNbrDev {
-    int n;
-    const double *x, *y, *z;
-    const long long *nbr_off;
-    const int *nbr;
-    int cap;
-    int use_cut;
-    double c2, Lx, Ly, Lz;
-    double nu;
-    double *fx, *fy, *fz;
-    double *energy;
-    unsigned long long *count;
-};
-
-template <class T, bool GRAD>
-__global__ __launch_bounds__(WAVE) void k_forces_fast_nbr(FastNbrDev a)
+// Cutoff algorithm: one wave per (owner, role) collects N(p) from the cells,
+// sorts it by index in LDS (bitonic), then runs that role's stream.  (Three
+// waves per owner rebuild the same list; the parallelism pays for that.)
+__global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
 {
-    extern __shared__ double smem[];
-    const int cap = a.cap;
-    double *sx = smem;
-    double *sy = sx + cap;
-    double *sz = sy + cap;
-    double *su = sz + cap;
-    int *sid = reinterpret_cast<int *>(su + cap);
-    unsigned *ring = reinterpret_cast<unsigned *>(sid + cap);
-
-    const int p = blockIdx.x;
+    extern __shared__ int sid_ex[];
+    __shared__ ExactShared sh;
+    __shared__ int sh_start[32], sh_pre[32];
+    // block b: role b % 3 of the owner in slot owner_slot(b / 3)
+    const int role = (int)(blockIdx.x % 3);
+    const int slot = owner_slot((int)(blockIdx.x / 3), a.n);
+    if (slot >= a.n) return;
+    const int p = a.cell_idx[slot];
     const int lane = lane_id();
-    const long long base = a.nbr_off[p];
-    const int len = (int)(a.nbr_off[p + 1] - base);
+    const int cap = a.cap;
     const double xp = a.x[p], yp = a.y[p], zp = a.z[p];
-    for (int t = lane; t < len; t += WAVE) {
-        const int q = a.nbr[base + t];
-        const double qx = a.x[q], qy = a.y[q], qz = a.z[q];
-        sx[t] = qx;
-        sy[t] = qy;
-        sz[t] = qz;
-        const double dx = qx - xp, dy = qy - yp, dz = qz - zp;
-        su[t] = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-        sid[t] = q;
-    }
-    __syncthreads();
 
-    const T nu = (T)a.nu, K = (T)(0.375 * a.nu);
-    double gx = 0.0, gy = 0.0, gz = 0.0, esum = 0.0;
-    unsigned cnt = 0;
-    int head = 0, qn = 0;
-
-    auto flush = [&](int nq) {
-        __syncthreads();
-        if (lane < nq) {
-            const unsigned e = ring[(head + lane) & (RING - 1)];
-            const int ia = (int)(e & 0xffffu), ib = (int)(e >> 16);
-            const double djx = sx[ia] - xp, djy = sy[ia] - yp, djz = sz[ia] - zp;
-            const double dkx = sx[ib] - xp, dky = sy[ib] - yp, dkz = sz[ib] - zp;
-            const double ex = sx[ib] - sx[ia], ey = sy[ib] - sy[ia], ez = sz[ib] - sz[ia];
-            T u, v, w;
-            if constexpr (sizeof(T) == 8) {
-                u = su[ia];
-                v = su[ib];
-                w = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-            } else {
-                u = (T)su[ia];
-                v = (T)su[ib];
-                const T fx_ = (T)ex, fy_ = (T)ey, fz_ = (T)ez;
-                w = fma_t<T>(fx_, fx_, fma_t<T>(fy_, fy_, fz_ * fz_));
-            }
-            T eu, ev, ew, et;
-            at_coeffs<T, false>(u, v, w, nu, K, eu, ev, ew, et);
-            if (GRAD) {
-                const double du = -(double)eu, dv = -(double)ev;
-                gx = __builtin_fma(du, djx, __builtin_fma(dv, dkx, gx));
-                gy = __builtin_fma(du, djy, __builtin_fma(dv, dky, gy));
-                gz = __builtin_fma(du, djz, __builtin_fma(dv, dkz, gz));
-            }
-            if (p < sid[ia] && p < sid[ib]) {
-                esum += (double)et;
-                ++cnt;
-            }
-        }
-        __syncthreads();
-        head = (head + nq) & (RING - 1);
-        qn -= nq;
-    };
-
-    const int H = (len - 1) / 2;
-    const long long full = (long long)H * len;
-    const long long total = (long long)len * (len - 1) / 2;
-    const float inv_len = len > 0 ? 1.0f / (float)len : 0.0f;
-    for (long long t0 = 0; t0 < total; t0 += WAVE) {
-        const long long t = t0 + lane;
-        bool ok = t < total;
-        int ia = 0, ib = 0;
+    const Candidates cand = gather_candidates(a, xp, yp, zp, sh_start, sh_pre);
+    int len = 0;
+    for (int t0 = 0; t0 < cand.total; t0 += WAVE) {
+        const int t = t0 + lane;
+        bool ok = t < cand.total;
+        int q = 0;
         if (ok) {
-            if (t < full) {
-                // t = d * len + r with 0 <= r < len: float estimate, exact fix-up
-                int d = (int)((float)t * inv_len);
-                long long r = t - (long long)d * len;
-                while (r < 0) {
-                    --d;
-                    r += len;
-                }
-                while (r >= len) {
-                    ++d;
-                    r -= len;
-                }
-                ia = (int)r;
-                ib = ia + d + 1;
-                if (ib >= len) ib -= len;
-            } else {
-                ia = (int)(t - full);
-                ib = ia + len / 2;
-            }
-            if (a.use_cut) {
-                const double d2 = sqr_dist_periodic(sx[ia], sy[ia], sz[ia], sx[ib], sy[ib], sz[ib], a.Lx, a.Ly, a.Lz);
-                ok = !(d2 > a.c2);
-            }
+            q = candidate(a, cand, sh_start, sh_pre, t);
+            ok = q != p && within_cutoff(a, xp, yp, zp, a.x[q], a.y[q], a.z[q]);
         }
         const unsigned long long m = __ballot(ok);
-        if (ok) ring[(head + qn + __popcll(m & __lanemask_lt())) & (RING - 1)] = (unsigned)ia | ((unsigned)ib << 16);
-        qn += __popcll(m);
-        if (qn >= WAVE) flush(WAVE);
+        const int pos = len + __popcll(m & __lanemask_lt());
+        if (ok && pos < cap) sid_ex[pos] = q;
+        len += __popcll(m);
     }
-    if (qn > 0) flush(qn);
-
-    if (GRAD) {
-        gx = wave_sum(gx);
-        gy = wave_sum(gy);
-        gz = wave_sum(gz);
-        if (lane == 0) {
-            a.fx[p] = gx;
-            a.fy[p] = gy;
-            a.fz[p] = gz;
+    if (lane == 0) note_list_length(a.st, len, cap);
+    if (len > cap) {
+        if (lane == 0) atomicOr(&a.st->flags[2], 1);
+        return;
+    }
+    // bitonic sort of sid_ex[0, P), padded with INT_MAX
+    int P = WAVE;
+    while (P < len) P <<= 1;
+    for (int t = len + lane; t < P; t += WAVE) sid_ex[t] = INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = lane; t < P; t += WAVE) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const int va = sid_ex[t], vb = sid_ex[u];
+                    if ((va > vb) == ((t & k) == 0)) {
+                        sid_ex[t] = vb;
+                        sid_ex[u] = va;
+                    }
+                }
+            }
+            __syncthreads();
         }
     }
-    esum = wave_sum(esum);
-    const unsigned long long c = wave_sum_u64(cnt);
-    if (lane == 0) {
-        if (a.energy && esum != 0.0) unsafeAtomicAdd(a.energy, esum);
-        if (c) atomicAdd(a.count, c);
-    }
+    int below = 0;
+    for (int t = lane; t < len; t += WAVE) below += sid_ex[t] < p ? 1 : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o);
+    const int split = below;
+
+    const LdsList list{sid_ex};
+    double acc = 0.0;
+    unsigned long long accepted = 0;
+    if (role == 0)
+        exact_role<0>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc, accepted);
+    else if (role == 1)
+        exact_role<1>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc, accepted);
+    else
+        exact_role<2>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc, accepted);
+    if (lane < 3) a.facc[(long long)(3 * role + lane) * a.n + p] = acc;
+    if (role == 0 && lane == 0) add_count(a.st, 0, accepted);
 }
 
 // ---------------------------------------------------------------------------
@@ -682,8 +817,9 @@ struct TileDev {
     const unsigned long long *triples;
     double nu;
     double *fx, *fy, *fz;
-    double *energy;
-    unsigned long long *count;
+    Status *st;
+    int count_slot;
+    int want_energy;
 };
 
 __device__ __forceinline__ void unpack_triple(unsigned long long t, int &t0, int &t1, int &t2)
@@ -749,10 +885,8 @@ __global__ __launch_bounds__(WAVE) void k_forces_tiles(TileDev a)
                 w = __builtin_fma(djkx, djkx, __builtin_fma(djky, djky, djkz * djkz));
             } else {
                 u = (T)u64;
-                const T ax_ = (T)dikx, ay_ = (T)diky, az_ = (T)dikz;
-                const T bx_ = (T)djkx, by_ = (T)djky, bz_ = (T)djkz;
-                v = fma_t<T>(ax_, ax_, fma_t<T>(ay_, ay_, az_ * az_));
-                w = fma_t<T>(bx_, bx_, fma_t<T>(by_, by_, bz_ * bz_));
+                v = (T)__builtin_fma(dikx, dikx, __builtin_fma(diky, diky, dikz * dikz));
+                w = (T)__builtin_fma(djkx, djkx, __builtin_fma(djky, djky, djkz * djkz));
             }
             T eu, ev, ew, et;
             at_coeffs<T, true>(u, v, w, nu, K, eu, ev, ew, et);
@@ -805,11 +939,11 @@ __global__ __launch_bounds__(WAVE) void k_forces_tiles(TileDev a)
             unsafeAtomicAdd(&a.fz[k], fkz[lane]);
         }
     }
-    esum = wave_sum(esum);
+    if (a.want_energy) esum = wave_sum(esum);
     const unsigned long long c = wave_sum_u64(cnt);
     if (lane == 0) {
-        if (a.energy && esum != 0.0) unsafeAtomicAdd(a.energy, esum);
-        if (c) atomicAdd(a.count, c);
+        if (a.want_energy) add_energy(a.st, esum);
+        add_count(a.st, a.count_slot, c);
     }
 }
 
@@ -822,16 +956,57 @@ __global__ void k_selftest_fp64(long long n, const double *__restrict__ a, const
     q[i] = a[i] / b[i];
 }
 
+GridDev grid_dev(const CellGrid &g)
+{
+    GridDev gd;
+    for (int c = 0; c < 3; ++c) {
+        gd.nc[c] = g.nc[c];
+        gd.lo[c] = g.lo[c];
+        gd.L[c] = g.L[c];
+    }
+    return gd;
+}
+
+CutDev cut_dev(const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu, int cap, Status *st,
+               int count_slot)
+{
+    CutDev a;
+    a.n = d.n;
+    a.x = d.x;
+    a.y = d.y;
+    a.z = d.z;
+    a.g = grid_dev(g);
+    a.cell_start = g.cell_start;
+    a.cell_idx = g.cell_idx;
+    a.c2 = c.c2;
+    a.Lx = c.L[0];
+    a.Ly = c.L[1];
+    a.Lz = c.L[2];
+    a.nu = nu;
+    a.cap = cap;
+    a.fx = d.fx;
+    a.fy = d.fy;
+    a.fz = d.fz;
+    a.facc = d.facc;
+    a.st = st;
+    a.count_slot = count_slot;
+    a.want_energy = 0;
+    return a;
+}
+
+unsigned owner_grid(int n) { return 8u * (unsigned)((n + 7) / 8); }
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Launchers.
 
-void launch_selftest_fp64(hipStream_t s, long long n, const double *a, const double *b, double *sq, double *q)
+size_t p3bca_fast_lds(int cap)
 {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_selftest_fp64, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, b, sq, q);
+    return (size_t)cap * (6 * sizeof(double) + sizeof(int)) + RING * sizeof(unsigned) + 64 * sizeof(int);
 }
+
+size_t p3bca_exact_lds(int cap) { return (size_t)cap * sizeof(int); }
 
 void launch_soa_from_rows(hipStream_t s, int n, const double *rows, double *x, double *y, double *z)
 {
@@ -845,16 +1020,23 @@ void launch_rows_from_soa(hipStream_t s, int n, const double *x, const double *y
     hipLaunchKernelGGL(k_rows_from_soa, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, y, z, rows);
 }
 
+void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite)
+{
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, s, st, keep_nonfinite);
+}
+
+void launch_finalize(hipStream_t s, Status *st) { hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NSLOT), 0, s, st); }
+
 void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h)
 {
     if (d.n <= 0) return;
     hipLaunchKernelGGL(k_predict, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, dt, h);
 }
 
-void launch_update(hipStream_t s, const DeviceState &d, int sum_roles, double hdt, const double g[3])
+void launch_update(hipStream_t s, const DeviceState &d, double hdt, const double g[3], const Status *st)
 {
     if (d.n <= 0) return;
-    hipLaunchKernelGGL(k_update, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, sum_roles, hdt, g[0], g[1], g[2]);
+    hipLaunchKernelGGL(k_update, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, hdt, g[0], g[1], g[2], st);
 }
 
 void launch_sum_roles(hipStream_t s, const DeviceState &d)
@@ -881,145 +1063,86 @@ long long scan_tmp_elems(long long count)
     return total + 64;
 }
 
-void launch_scan_i32_to_i64(hipStream_t s, const int *in, long long *out, long long count, long long *tmp)
-{
-    scan_rec<int, long long>(s, in, count, out, count + 1, tmp);
-}
-
 void launch_scan_i32(hipStream_t s, const int *in, int *out, int count, long long *tmp)
 {
     scan_rec<int, int>(s, in, count, out, (long long)count + 1, tmp);
 }
 
-void launch_cells_build(hipStream_t s, int n, const double *x, const double *y, const double *z, CellGrid &g,
-                        int *flags, long long *tmp)
+void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double dt, double h, int zero_f,
+                        CellGrid &g, Status *st, long long *tmp)
 {
-    GridDev gd;
-    for (int c = 0; c < 3; ++c) {
-        gd.nc[c] = g.nc[c];
-        gd.lo[c] = g.lo[c];
-        gd.L[c] = g.L[c];
-    }
-    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(g.ncell, 256)), dim3(256), 0, s, g.cell_cnt, (long long)g.ncell);
-    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(g.ncell, 256)), dim3(256), 0, s, g.cell_cur, (long long)g.ncell);
+    const int n = d.n;
     if (n > 0)
-        hipLaunchKernelGGL(k_cell_assign, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, y, z, gd, g.cell_of,
-                           g.cell_cnt, flags);
+        hipLaunchKernelGGL(k_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, predict, dt, h, zero_f, grid_dev(g),
+                           g.cell_of, g.cell_cnt, st);
     launch_scan_i32(s, g.cell_cnt, g.cell_start, g.ncell, tmp);
     if (n > 0)
         hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, g.cell_of, g.cell_start,
-                           g.cell_cur, g.cell_idx);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(g.ncell, 64)), dim3(64), 0, s, g.ncell, g.cell_start, g.cell_idx);
+                           g.cell_cnt, g.cell_idx);
 }
 
-static NbrDev make_nbr_dev(int n, const double *x, const double *y, const double *z, const CellGrid &g,
-                           const CutoffSpec &c)
+template <class K>
+static void allow_lds(K kernel, size_t bytes)
 {
-    NbrDev a;
-    a.n = n;
-    a.x = x;
-    a.y = y;
-    a.z = z;
-    for (int k = 0; k < 3; ++k) {
-        a.g.nc[k] = g.nc[k];
-        a.g.lo[k] = g.lo[k];
-        a.g.L[k] = g.L[k];
-    }
-    a.cell_of = g.cell_of;
-    a.cell_start = g.cell_start;
-    a.cell_idx = g.cell_idx;
-    a.use_cut = c.use_cut;
-    a.c2 = c.c2;
-    a.Lx = c.L[0];
-    a.Ly = c.L[1];
-    a.Lz = c.L[2];
-    return a;
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bytes);
 }
 
-void launch_nbr_count(hipStream_t s, int n, const double *x, const double *y, const double *z, const CellGrid &g,
-                      const CutoffSpec &c, int brute, int *cnt, int *flags)
-{
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_nbr_count, dim3(blocks_for(n, 128)), dim3(128), 0, s, make_nbr_dev(n, x, y, z, g, c), brute,
-                       cnt, flags);
-}
-
-void launch_nbr_fill(hipStream_t s, int n, const double *x, const double *y, const double *z, const CellGrid &g,
-                     const CutoffSpec &c, int brute, const long long *off, int *nbr)
-{
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_nbr_fill, dim3(blocks_for(n, 128)), dim3(128), 0, s, make_nbr_dev(n, x, y, z, g, c), brute,
-                       off, nbr);
-}
-
-void launch_nbr_sort(hipStream_t s, int n, const long long *off, int *nbr, int maxlen)
-{
-    if (n <= 0 || maxlen <= 1) return;
-    int P = WAVE;
-    while (P < maxlen) P <<= 1;
-    hipLaunchKernelGGL(k_nbr_sort, dim3((unsigned)n), dim3(WAVE), (size_t)P * sizeof(int), s, n, off, nbr);
-}
-
-void launch_forces_exact(hipStream_t s, const DeviceState &d, const long long *nbr_off, const int *nbr,
-                         const CutoffSpec &c, double nu, unsigned long long *count)
+void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
+                       int cap, int mixed, int grad, Status *st, int count_slot, int want_energy)
 {
     if (d.n <= 0) return;
-    ExactDev a;
+    CutDev a = cut_dev(d, g, c, nu, cap, st, count_slot);
+    a.want_energy = want_energy;
+    const size_t shm = p3bca_fast_lds(cap);
+    const dim3 grid(owner_grid(d.n)), blk(WAVE);
+    if (mixed) {
+        if (grad) {
+            allow_lds(k_p3bca_fast<float, true>, shm);
+            hipLaunchKernelGGL((k_p3bca_fast<float, true>), grid, blk, shm, s, a);
+        } else {
+            allow_lds(k_p3bca_fast<float, false>, shm);
+            hipLaunchKernelGGL((k_p3bca_fast<float, false>), grid, blk, shm, s, a);
+        }
+    } else {
+        if (grad) {
+            allow_lds(k_p3bca_fast<double, true>, shm);
+            hipLaunchKernelGGL((k_p3bca_fast<double, true>), grid, blk, shm, s, a);
+        } else {
+            allow_lds(k_p3bca_fast<double, false>, shm);
+            hipLaunchKernelGGL((k_p3bca_fast<double, false>), grid, blk, shm, s, a);
+        }
+    }
+}
+
+void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
+                        int cap, Status *st)
+{
+    if (d.n <= 0) return;
+    const CutDev a = cut_dev(d, g, c, nu, cap, st, 0);
+    const size_t shm = p3bca_exact_lds(cap);
+    allow_lds(k_p3bca_exact, shm);
+    hipLaunchKernelGGL(k_p3bca_exact, dim3(3 * owner_grid(d.n)), dim3(WAVE), shm, s, a);
+}
+
+void launch_auta_exact(hipStream_t s, const DeviceState &d, double nu, Status *st)
+{
+    if (d.n <= 0) return;
+    AutaExactDev a;
     a.n = d.n;
     a.x = d.x;
     a.y = d.y;
     a.z = d.z;
-    a.nbr_off = nbr_off;
-    a.nbr = nbr;
-    a.use_cut = c.use_cut;
-    a.c2 = c.c2;
-    a.Lx = c.L[0];
-    a.Ly = c.L[1];
-    a.Lz = c.L[2];
     a.nu = nu;
     a.facc = d.facc;
-    a.count = count;
-    hipLaunchKernelGGL(k_forces_exact, dim3((unsigned)(3 * d.n)), dim3(WAVE), 0, s, a);
-}
-
-void launch_forces_fast_nbr(hipStream_t s, const DeviceState &d, const long long *nbr_off, const int *nbr,
-                            int maxlen, const CutoffSpec &c, double nu, int mixed, int grad, double *energy,
-                            unsigned long long *count)
-{
-    if (d.n <= 0) return;
-    FastNbrDev a;
-    a.n = d.n;
-    a.x = d.x;
-    a.y = d.y;
-    a.z = d.z;
-    a.nbr_off = nbr_off;
-    a.nbr = nbr;
-    a.cap = maxlen < 1 ? 1 : maxlen;
-    a.use_cut = c.use_cut;
-    a.c2 = c.c2;
-    a.Lx = c.L[0];
-    a.Ly = c.L[1];
-    a.Lz = c.L[2];
-    a.nu = nu;
-    a.fx = d.fx;
-    a.fy = d.fy;
-    a.fz = d.fz;
-    a.energy = energy;
-    a.count = count;
-    const size_t shm = (size_t)a.cap * (4 * sizeof(double) + sizeof(int)) + RING * sizeof(unsigned);
-    const dim3 grid((unsigned)d.n), blk(WAVE);
-    if (mixed) {
-        if (grad) hipLaunchKernelGGL((k_forces_fast_nbr<float, true>), grid, blk, shm, s, a);
-        else hipLaunchKernelGGL((k_forces_fast_nbr<float, false>), grid, blk, shm, s, a);
-    } else {
-        if (grad) hipLaunchKernelGGL((k_forces_fast_nbr<double, true>), grid, blk, shm, s, a);
-        else hipLaunchKernelGGL((k_forces_fast_nbr<double, false>), grid, blk, shm, s, a);
-    }
+    a.st = st;
+    hipLaunchKernelGGL(k_auta_exact, dim3((unsigned)(3 * d.n)), dim3(WAVE), 0, s, a);
 }
 
 void launch_forces_tiles(hipStream_t s, const DeviceState &d, const unsigned long long *triples,
-                         long long ntriples, double nu, int mixed, int grad, double *energy,
-                         unsigned long long *count)
+                         long long ntriples, double nu, int mixed, int grad, Status *st, int count_slot,
+                         int want_energy)
 {
     if (d.n <= 0 || ntriples <= 0) return;
     TileDev a;
@@ -1032,8 +1155,9 @@ void launch_forces_tiles(hipStream_t s, const DeviceState &d, const unsigned lon
     a.fx = d.fx;
     a.fy = d.fy;
     a.fz = d.fz;
-    a.energy = energy;
-    a.count = count;
+    a.st = st;
+    a.count_slot = count_slot;
+    a.want_energy = want_energy;
     const dim3 grid((unsigned)ntriples), blk(WAVE);
     if (mixed) {
         if (grad) hipLaunchKernelGGL((k_forces_tiles<float, true>), grid, blk, 0, s, a);
@@ -1042,6 +1166,12 @@ void launch_forces_tiles(hipStream_t s, const DeviceState &d, const unsigned lon
         if (grad) hipLaunchKernelGGL((k_forces_tiles<double, true>), grid, blk, 0, s, a);
         else hipLaunchKernelGGL((k_forces_tiles<double, false>), grid, blk, 0, s, a);
     }
+}
+
+void launch_selftest_fp64(hipStream_t s, long long n, const double *a, const double *b, double *sq, double *q)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_selftest_fp64, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, b, sq, q);
 }
 
 }  // namespace tb3
