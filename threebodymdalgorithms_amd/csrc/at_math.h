@@ -37,6 +37,23 @@ __device__ __forceinline__ double sqr_dist_periodic(double ax, double ay, double
     return ex * ex + ey * ey + ez * ez;
 }
 
+// The same value with one v_min_f64 per axis instead of a compare and two
+// selects.  It is bit-identical to sqr_dist_periodic for every input: t is
+// |b - a| >= +0 and L > 0, so t and L - t are never both zero (no +0/-0
+// tie), t is NaN exactly when L - t is (both operands NaN), and t = +inf gives
+// L - t = -inf, which both forms return.
+__device__ __forceinline__ double sqr_dist_periodic_min(double ax, double ay, double az, double bx, double by,
+                                                        double bz, double Lx, double Ly, double Lz)
+{
+    const double tx = __builtin_fabs(bx - ax);
+    const double ty = __builtin_fabs(by - ay);
+    const double tz = __builtin_fabs(bz - az);
+    const double ex = __builtin_fmin(tx, Lx - tx);
+    const double ey = __builtin_fmin(ty, Ly - ty);
+    const double ez = __builtin_fmin(tz, Lz - tz);
+    return ex * ex + ey * ey + ez * ez;
+}
+
 // Eigen's Vector3d::norm() as sqrt((x*x + y*y) + z*z) (AxilrodTeller.cpp:29-31).
 __device__ __forceinline__ double ref_norm3(double x, double y, double z)
 {

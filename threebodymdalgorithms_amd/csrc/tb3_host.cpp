@@ -113,7 +113,9 @@ struct tb3_ctx {
     int grid_cell_cap = 0, grid_n_cap = 0;
     long long *scan_tmp = nullptr;
     long long scan_tmp_cap = 0;
-    int list_cap = 128;  // LDS neighbour-list capacity per wave (power of two)
+    int list_cap = 128;    // LDS neighbour-list capacity per wave (power of two)
+    bool cells_ok = true;  // the cell-per-workgroup kernel staged every neighbourhood so far
+    int cells_grid = 1;    // its persistent grid (CUs x resident workgroups per CU)
 
     // tile triples for the all-triplets fast sweep
     unsigned long long *triples = nullptr;
@@ -217,6 +219,7 @@ void configure_grid(tb3_ctx *c)
         c->grid.lo[k] = c->lo[k];
         c->grid.L[k] = c->L[k];
     }
+    c->cells_ok = true;  // a new grid gets a new chance at the cell kernel
 }
 
 void ensure_grid_buffers(tb3_ctx *c, int n)
@@ -231,6 +234,9 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
     if (n > c->grid_n_cap) {
         dev_alloc(g.cell_of, n);
         dev_alloc(g.cell_idx, n);
+        dev_alloc(g.xs, n);
+        dev_alloc(g.ys, n);
+        dev_alloc(g.zs, n);
         c->grid_n_cap = n;
     }
     const long long need = scan_tmp_elems(std::max<long long>(n, g.ncell));
@@ -290,8 +296,12 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
         }
         if (fast_grad || energy) {
             const int slot = (exact && grad) ? 1 : 0;  // the exact pass owns count[0]
-            if (cut) launch_p3bca_fast(s, d, c->grid, cs, c->nu, c->list_cap, mixed, fast_grad ? 1 : 0, c->d_status,
-                                       slot, energy ? 1 : 0);
+            if (cut && c->cells_ok)
+                launch_p3bca_cells(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
+                                   energy ? 1 : 0, std::min(c->cells_grid, c->grid.ncell));
+            else if (cut)
+                launch_p3bca_fast(s, d, c->grid, cs, c->nu, c->list_cap, mixed, fast_grad ? 1 : 0, c->d_status,
+                                  slot, energy ? 1 : 0);
             else launch_forces_tiles(s, d, c->triples, c->ntriples, c->nu, mixed, fast_grad ? 1 : 0, c->d_status,
                                      slot, energy ? 1 : 0);
         }
@@ -336,12 +346,17 @@ void launch_prepare(tb3_ctx *c, hipStream_t s, DeviceState &d, bool predict, dou
 bool grow_after_overflow(tb3_ctx *c)
 {
     if (!c->h_status->flags[2]) return false;
+    // a neighbourhood too large for the cell kernel's staging: use the
+    // per-owner kernel for this particle system from now on
+    if (c->h_status->flags[3]) c->cells_ok = false;
     const int need = c->h_status->flags[1];
-    if (need > MAX_LIST)
-        throw Fail(TB3_EUNSUPPORTED, "a particle has " + std::to_string(need) +
-                                         " neighbours within the cutoff; at most " + std::to_string(MAX_LIST) +
-                                         " are supported");
-    c->list_cap = std::min(next_pow2(need + need / 8), MAX_LIST);
+    if (need > c->list_cap) {
+        if (need > MAX_LIST)
+            throw Fail(TB3_EUNSUPPORTED, "a particle has " + std::to_string(need) +
+                                             " neighbours within the cutoff; at most " + std::to_string(MAX_LIST) +
+                                             " are supported");
+        c->list_cap = std::min(next_pow2(need + need / 8), MAX_LIST);
+    }
     return true;
 }
 
@@ -370,7 +385,8 @@ unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool inte
     if (integrate) launch_update(s, d, dt * 0.5, g, c->d_status);
     if (c->recording) HIPCHK(hipEventRecord(c->ev[4], s));
     read_status(c, s);
-    while (grow_after_overflow(c)) {
+    for (int retry = 0; grow_after_overflow(c); ++retry) {
+        if (retry >= 8) throw Fail(TB3_EHIP, "the cutoff sweep kept overflowing its LDS staging");
         c->recording = false;
         launch_reset_status(s, c->d_status, 1);  // keeps the non-finite flag of the binning pass
         if (grad && !exact) {
@@ -439,6 +455,9 @@ tb3_status tb3_create(int device, tb3_ctx **out)
     c->device = device;
     tb3_status st = guarded(c, [&] {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        int ncu = 0;
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        c->cells_grid = std::max(1, ncu) * p3bca_cells_blocks_per_cu();
         dev_alloc(c->d_slots, 1);
         HIPCHK(hipMemset(c->d_slots, 0, sizeof(Slots)));
         dev_alloc(c->d_status, 1);

@@ -30,6 +30,7 @@ struct CellGrid {
     int *cell_cnt = nullptr;    // ncell (zero between steps)
     int *cell_start = nullptr;  // ncell + 1
     int *cell_idx = nullptr;    // n, particle indices grouped by cell (order within a cell unspecified)
+    double *xs = nullptr, *ys = nullptr, *zs = nullptr;  // n, positions in cell_idx order
 };
 
 struct CutoffSpec {
@@ -53,7 +54,10 @@ struct Status {
     unsigned long long count[2];  // [0] accepted triplets, [1] discarded count of an auxiliary pass
     double energy;
     int flags[4];                 // [0] non-finite position seen, [1] longest neighbour list (recorded
-                                  // when above half the capacity), [2] list overflow
+                                  // when above half the capacity), [2] list overflow (the update
+                                  // kernel skips itself), [3] the cell kernel could not stage a
+                                  // neighbourhood (re-run with the per-owner kernel)
+    int queue[2];                 // cell queue of the cell kernel; returns to zero after each launch
     Slots *slots;                 // partial sums of the sweep kernels
 };
 
@@ -93,6 +97,14 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
                        int cap, int mixed, int grad, Status *st, int count_slot, int want_energy);
 void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
                         int cap, Status *st);
+// Fast cutoff sweep with one cell per workgroup (persistent grid of `grid`
+// workgroups over a device queue of cells).  Sets st->flags[2] and [3] when a
+// neighbourhood does not fit its LDS staging; the caller then re-runs the
+// sweep with launch_p3bca_fast.
+void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid);
+// Workgroups of launch_p3bca_cells that fit on one CU at once.
+int p3bca_cells_blocks_per_cu();
 
 // All-triplet sweeps (NATA / AUTA, or P3BCA without a cutoff).
 void launch_auta_exact(hipStream_t s, const DeviceState &d, double nu, Status *st);

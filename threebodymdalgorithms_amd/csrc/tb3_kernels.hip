@@ -31,6 +31,38 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int lane_prefix(unsigned long long m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes
+// (LDS operations of one wave execute in order; this keeps the compiler from
+// moving them across the point).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Unordered pair number t of {0 .. len-1} in folded order: t = d len + r
+// (d < H = (len-1)/2) pairs r with (r + d + 1) mod len; for even len the last
+// len/2 pairs are (r, r + len/2).  d = floor((t + 1/2) / len) in FP64 is exact
+// (the quotient is 1/(2 len) away from an integer, far above the rounding).
+__device__ __forceinline__ void folded_pair(int t, int len, int full, double inv_len, int &ia, int &ib)
+{
+    const int d = (int)(((double)t + 0.5) * inv_len);
+    const int r = t - d * len;
+    const int b1 = r + d + 1;
+    const int ib_full = b1 >= len ? b1 - len : b1;
+    const int ia_half = t - full;
+    const bool in_full = t < full;
+    ia = in_full ? r : ia_half;
+    ib = in_full ? ib_full : ia_half + (len >> 1);
+}
+
 __device__ __forceinline__ void lds_add(double *p, double v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -292,16 +324,23 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
     atomicAdd(&cell_cnt[c], 1);
 }
 
-// Places each particle in its cell's range, counting the cell's counter back
-// down to zero so the counters are clear for the next step.
-__global__ void k_cell_scatter(int n, const int *__restrict__ cell_of, const int *__restrict__ cell_start,
-                               int *__restrict__ cell_cnt, int *__restrict__ cell_idx)
+// Places each particle in its cell's range (index and a cell-ordered copy of
+// its position), counting the cell's counter back down to zero so the
+// counters are clear for the next step.
+__global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
+                               const double *__restrict__ z, const int *__restrict__ cell_of,
+                               const int *__restrict__ cell_start, int *__restrict__ cell_cnt,
+                               int *__restrict__ cell_idx, double *__restrict__ xs, double *__restrict__ ys,
+                               double *__restrict__ zs)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int c = cell_of[p];
-    const int slot = atomicSub(&cell_cnt[c], 1) - 1;
-    cell_idx[cell_start[c] + slot] = p;
+    const int slot = cell_start[c] + atomicSub(&cell_cnt[c], 1) - 1;
+    cell_idx[slot] = p;
+    xs[slot] = x[p];
+    ys[slot] = y[p];
+    zs[slot] = z[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -583,6 +622,279 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
     if (lane == 0) {
         if (a.want_energy) add_energy(a.st, esum);
         add_count(a.st, a.count_slot, c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fast cutoff sweep, one cell per workgroup (the default fast kernel).
+//
+// Ownership: each accepted triplet belongs to the one particle that sees both
+// others "forward": in its own cell at a later slot, or in a forward
+// neighbour cell (the half shell of positive stencil offsets, or of larger
+// cell index on grids with a 3-cell dimension; see below).  A workgroup takes
+// cell c from a device queue and stages in LDS c's particles followed by
+// those of c's forward cells: every particle forward of an owner in c and
+// within the cutoff of it is in this range.  Each
+// wave then claims owners i of cell c, collects i's candidates t > i within
+// the cutoff (a contiguous LDS range, no search), walks the unordered pairs
+// of that list in folded order, tests the third side and evaluates 64 queued
+// triplets at a time.  All three gradients go to per-particle LDS
+// accumulators (LDS atomics); the workgroup adds them to global memory once
+// per staged particle.  A neighbourhood larger than NBCAP, or an owner list
+// larger than LCAP, raises the overflow flags and the host falls back to the
+// per-owner kernel above.
+
+constexpr int CELL_WAVES = 4;
+constexpr int NBCAP = 512;
+constexpr int LCAP = 256;
+
+struct CellsDev {
+    CutDev c;
+    const double *xs, *ys, *zs;
+    int ncell;
+    int *queue;  // [0] next cell ticket, [1] finished workgroups (both return to 0)
+};
+
+template <class T, bool GRAD>
+__global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
+{
+    __shared__ double nx[NBCAP], ny[NBCAP], nz[NBCAP];
+    __shared__ double gxs[NBCAP], gys[NBCAP], gzs[NBCAP];
+    __shared__ int nid[NBCAP];
+    __shared__ unsigned short wl[CELL_WAVES][LCAP];
+    __shared__ unsigned ring[CELL_WAVES][RING];
+    __shared__ int cst[32], cpre[33];
+    __shared__ int s_cell, s_total, s_own, s_ncells, s_next;
+
+    const CutDev &A = a.c;
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid / WAVE;
+    const T nu = (T)A.nu, K = (T)(0.375 * A.nu);
+    const double Lx = A.Lx, Ly = A.Ly, Lz = A.Lz, c2 = A.c2;
+    const int ncx = A.g.nc[0], ncy = A.g.nc[1], ncz = A.g.nc[2];
+    double esum = 0.0;
+    unsigned cnt = 0;
+    // Non-finite positions pair with everything: leave them to the per-owner
+    // kernel.  (The flag was set by the binning pass, so every thread reads
+    // the same value.)  The cell loop below leaves only on conditions that
+    // are uniform across the workgroup; a wave's list overflow just marks it.
+    const bool skip = A.st->flags[0] != 0;
+    bool overflow = skip;
+
+    while (!skip) {
+        if (tid == 0) s_cell = atomicAdd(&a.queue[0], 1);
+        __syncthreads();
+        const int c = s_cell;
+        if (c >= a.ncell) break;
+
+        // 1. the staged cells: c first, then neighbour cells with a larger index
+        if (wv == 0) {
+            const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+            const int wx = ncx >= 3 ? 3 : 1, wy = ncy >= 3 ? 3 : 1, wz = ncz >= 3 ? 3 : 1;
+            const int nst = wx * wy * wz;
+            int cc = -1;
+            if (lane < nst) {
+                const int ox = lane % wx, oy = (lane / wx) % wy, oz = lane / (wx * wy);
+                int xx = cx + (wx == 3 ? ox - 1 : 0), yy = cy + (wy == 3 ? oy - 1 : 0);
+                int zz = cz + (wz == 3 ? oz - 1 : 0);
+                xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
+                yy = yy < 0 ? yy + ncy : (yy >= ncy ? yy - ncy : yy);
+                zz = zz < 0 ? zz + ncz : (zz >= ncz ? zz - ncz : zz);
+                cc = (zz * ncy + yy) * ncx + xx;
+            }
+            // Which neighbour cells are "forward".  With >= 4 cells (or 1) in
+            // every dimension, pairwise-adjacent cells lie in one unwrapped
+            // 2x2x2 block, so orienting each pair of cells by the
+            // lexicographic sign of the stencil offset (dz, dy, dx) is
+            // acyclic on every triplet: the 13 positive offsets (the half
+            // shell) give each triplet exactly one owner.  With exactly 3
+            // cells in a dimension three cells can wrap into a cycle, so the
+            // cell index orders them instead.
+            const bool lexi = (ncx == 1 || ncx >= 4) && (ncy == 1 || ncy >= 4) && (ncz == 1 || ncz >= 4);
+            const int centre = nst / 2;
+            const bool fwd = cc >= 0 && (lexi ? lane > centre : cc > c);
+            const int own_start = A.cell_start[c];
+            const int own_cnt = A.cell_start[c + 1] - own_start;
+            int st = 0, n_in = 0;
+            if (fwd) {
+                st = A.cell_start[cc];
+                n_in = A.cell_start[cc + 1] - st;
+            }
+            const unsigned long long m = __ballot(fwd);
+            int incl = n_in;
+#pragma unroll
+            for (int o = 1; o < WAVE; o <<= 1) {
+                const int y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            const int nfwd = __popcll(m);
+            if (fwd) {
+                const int k = 1 + lane_prefix(m);
+                cst[k] = st;
+                cpre[k] = own_cnt + incl - n_in;
+            }
+            if (lane == 0) {
+                cst[0] = own_start;
+                cpre[0] = 0;
+                s_own = own_cnt;
+                s_ncells = 1 + nfwd;
+            }
+            const int total = own_cnt + __shfl(incl, WAVE - 1);
+            if (lane == 0) {
+                cpre[1 + nfwd] = total;
+                s_total = total;
+            }
+        }
+        __syncthreads();
+        const int total = s_total, own = s_own, nk = s_ncells;
+        if (total > NBCAP) {
+            overflow = true;
+            break;
+        }
+
+        // 2. stage positions and clear the accumulators
+        for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
+            int k = 0;
+            while (k + 1 < nk && cpre[k + 1] <= t) ++k;
+            const int slot = cst[k] + (t - cpre[k]);
+            nx[t] = a.xs[slot];
+            ny[t] = a.ys[slot];
+            nz[t] = a.zs[slot];
+            nid[t] = A.cell_idx[slot];
+            gxs[t] = 0.0;
+            gys[t] = 0.0;
+            gzs[t] = 0.0;
+        }
+        if (tid == 0) s_next = 0;
+        __syncthreads();
+
+        // 3. owners of cell c, claimed one at a time by the waves
+        unsigned short *list = wl[wv];
+        unsigned *rg = ring[wv];
+        for (;;) {
+            int i = 0;
+            if (lane == 0) i = atomicAdd(&s_next, 1);
+            i = __shfl(i, 0);
+            if (i >= own) break;
+            const double xi = nx[i], yi = ny[i], zi = nz[i];
+
+            int len = 0;
+            for (int t0 = i + 1; t0 < total; t0 += WAVE) {
+                const int t = t0 + lane;
+                const bool ok = t < total &&
+                                !(sqr_dist_periodic_min(xi, yi, zi, nx[t], ny[t], nz[t], Lx, Ly, Lz) > c2);
+                const unsigned long long m = __ballot(ok);
+                const int pos = len + lane_prefix(m);
+                if (ok && pos < LCAP) list[pos] = (unsigned short)t;
+                len += __popcll(m);
+            }
+            if (len > LCAP) {
+                overflow = true;
+                continue;
+            }
+            wave_sync();
+
+            double gix = 0.0, giy = 0.0, giz = 0.0;
+            int head = 0, qn = 0;
+            auto flush = [&](int nq) {
+                wave_sync();
+                if (lane < nq) {
+                    const unsigned e = rg[(head + lane) & (RING - 1)];
+                    const int j = (int)(e & 0xffffu), k = (int)(e >> 16);
+                    const double xj = nx[j], yj = ny[j], zj = nz[j];
+                    const double xk = nx[k], yk = ny[k], zk = nz[k];
+                    const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
+                    const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
+                    const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
+                    const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                    const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                    const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+                    T eu, ev, ew, et;
+                    at_coeffs<T, GRAD>((T)u64, (T)v64, (T)w64, nu, K, eu, ev, ew, et);
+                    if (GRAD) {
+                        const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
+                        gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
+                        giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
+                        giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
+                        lds_add(&gxs[j], __builtin_fma(cu, djx, -cw * ex));
+                        lds_add(&gys[j], __builtin_fma(cu, djy, -cw * ey));
+                        lds_add(&gzs[j], __builtin_fma(cu, djz, -cw * ez));
+                        lds_add(&gxs[k], __builtin_fma(cv, dkx, cw * ex));
+                        lds_add(&gys[k], __builtin_fma(cv, dky, cw * ey));
+                        lds_add(&gzs[k], __builtin_fma(cv, dkz, cw * ez));
+                    }
+                    esum += (double)et;
+                    ++cnt;
+                }
+                wave_sync();
+                head = (head + nq) & (RING - 1);
+                qn -= nq;
+            };
+
+            const int H = (len - 1) / 2;
+            const int full = H * len;
+            const int npairs = len * (len - 1) / 2;
+            const double inv_len = len > 0 ? 1.0 / (double)len : 0.0;
+            for (int t0 = 0; t0 < npairs; t0 += WAVE) {
+                const int t = t0 + lane;
+                bool ok = t < npairs;
+                int j = 0, k = 0;
+                if (ok) {
+                    int ia, ib;
+                    folded_pair(t, len, full, inv_len, ia, ib);
+                    j = list[ia];
+                    k = list[ib];
+                    ok = !(sqr_dist_periodic_min(nx[j], ny[j], nz[j], nx[k], ny[k], nz[k], Lx, Ly, Lz) > c2);
+                }
+                const unsigned long long m = __ballot(ok);
+                if (ok) rg[(head + qn + lane_prefix(m)) & (RING - 1)] = (unsigned)j | ((unsigned)k << 16);
+                qn += __popcll(m);
+                if (qn >= WAVE) flush(WAVE);
+            }
+            if (qn > 0) flush(qn);
+            if (GRAD) {
+                gix = wave_sum(gix);
+                giy = wave_sum(giy);
+                giz = wave_sum(giz);
+                if (lane == 0) {
+                    lds_add(&gxs[i], gix);
+                    lds_add(&gys[i], giy);
+                    lds_add(&gzs[i], giz);
+                }
+            }
+        }
+        __syncthreads();
+
+        // 4. add the staged particles' gradients to global memory
+        if (GRAD) {
+            for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
+                const int q = nid[t];
+                unsafeAtomicAdd(&A.fx[q], gxs[t]);
+                unsafeAtomicAdd(&A.fy[q], gys[t]);
+                unsafeAtomicAdd(&A.fz[q], gzs[t]);
+            }
+        }
+        __syncthreads();
+    }
+
+    if (overflow && lane == 0) {
+        atomicOr(&A.st->flags[2], 1);  // the update kernel skips itself
+        atomicOr(&A.st->flags[3], 1);  // the host re-runs the sweep with the per-owner kernel
+    }
+    if (A.want_energy) esum = wave_sum(esum);
+    const unsigned long long cw = wave_sum_u64(cnt);
+    if (lane == 0) {
+        if (A.want_energy) add_energy(A.st, esum);
+        add_count(A.st, A.count_slot, cw);
+    }
+    // the last workgroup to finish returns the queue to zero for the next launch
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        const int done = atomicAdd(&a.queue[1], 1);
+        if (done == (int)gridDim.x - 1) {
+            atomicExch(&a.queue[0], 0);
+            atomicExch(&a.queue[1], 0);
+        }
     }
 }
 
@@ -1077,8 +1389,8 @@ void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double
                            g.cell_of, g.cell_cnt, st);
     launch_scan_i32(s, g.cell_cnt, g.cell_start, g.ncell, tmp);
     if (n > 0)
-        hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, g.cell_of, g.cell_start,
-                           g.cell_cnt, g.cell_idx);
+        hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d.x, d.y, d.z, g.cell_of,
+                           g.cell_start, g.cell_cnt, g.cell_idx, g.xs, g.ys, g.zs);
 }
 
 template <class K>
@@ -1114,6 +1426,37 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
             hipLaunchKernelGGL((k_p3bca_fast<double, false>), grid, blk, shm, s, a);
         }
     }
+}
+
+void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid)
+{
+    if (d.n <= 0) return;
+    CellsDev a;
+    a.c = cut_dev(d, g, c, nu, 0, st, count_slot);
+    a.c.want_energy = want_energy;
+    a.xs = g.xs;
+    a.ys = g.ys;
+    a.zs = g.zs;
+    a.ncell = g.ncell;
+    a.queue = &st->queue[0];
+    const dim3 gr((unsigned)(grid < 1 ? 1 : grid)), blk(CELL_WAVES * WAVE);
+    if (mixed) {
+        if (grad) hipLaunchKernelGGL((k_p3bca_cells<float, true>), gr, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_p3bca_cells<float, false>), gr, blk, 0, s, a);
+    } else {
+        if (grad) hipLaunchKernelGGL((k_p3bca_cells<double, true>), gr, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_p3bca_cells<double, false>), gr, blk, 0, s, a);
+    }
+}
+
+int p3bca_cells_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(k_p3bca_cells<double, true>),
+                                                     CELL_WAVES * WAVE, 0) != hipSuccess || nb < 1)
+        nb = 1;
+    return nb;
 }
 
 void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
