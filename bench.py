@@ -207,7 +207,7 @@ def main():
                    "integrate_ms": fp64["integrate_ms"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_forces_fast_nbr<double>", "flop_per_triplet": FLOP_PER_TRIPLET},
+                     "kernel": "tb3::k_p3bca_cells<double, true>", "flop_per_triplet": FLOP_PER_TRIPLET},
         "cpu_baseline": cpu,
         "exact": out_exact,
         "fp32_mixed": out_fp32,

@@ -148,10 +148,12 @@ tb3_status tb3_set_profiling(tb3_ctx *ctx, int enable);
 tb3_status tb3_get_profile(const tb3_ctx *ctx, double *out, int64_t n);
 
 /* Diagnostic: element-wise sqrt(a) and a / b in device FP64, as the
- * bit-exact path evaluates them, for checking that both round correctly.
- * Host arrays of n doubles.  No counterpart in the reference. */
+ * bit-exact path evaluates them, for checking that both round correctly, and
+ * the raw hardware reciprocal-square-root estimate of a (v_rsq_f64, the seed
+ * of the fast path's 1/sqrt; rsq_a may be NULL).  Host arrays of n doubles.
+ * No counterpart in the reference. */
 tb3_status tb3_selftest_fp64(tb3_ctx *ctx, int64_t n, const double *a, const double *b, double *sqrt_a,
-                             double *a_div_b);
+                             double *a_div_b, double *rsq_a);
 
 #ifdef __cplusplus
 }

@@ -68,9 +68,13 @@ def test_fp64_sqrt_and_division_round_correctly(eng):
     b = np.concatenate([rng.uniform(0.5, 2, n // 2), 10.0 ** rng.uniform(-20, 20, n // 2)]) * \
         rng.choice([-1.0, 1.0], n)
     say(f"selftest_fp64 n={n}")
-    sq, q = eng.selftest_fp64(a, b)
+    sq, q, rs = eng.selftest_fp64(a, b)
     assert np.array_equal(sq, np.sqrt(a)), int(np.sum(sq != np.sqrt(a)))
     assert np.array_equal(q, a / b), int(np.sum(q != a / b))
+    # the hardware 1/sqrt estimate seeds the fast path's Newton steps
+    rel = np.abs(rs * np.sqrt(a) - 1.0)
+    say(f"  v_rsq_f64 max relative error {rel.max():.3e} (2^{np.log2(max(rel.max(), 1e-300)):.1f})")
+    assert rel.max() < 1e-6
 
 
 @pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 150, 4.0, 0.0), ("nata", 97, 4.0, 0.0),

@@ -172,6 +172,38 @@ __device__ __forceinline__ float fma_t<float>(float a, float b, float c)
 __device__ __forceinline__ double rsqrt_t(double x) { return rsqrt_f64(x); }
 __device__ __forceinline__ float rsqrt_t(float x) { return rsqrt_f32(x); }
 
+// Mixed-precision form: u, v, w arrive in FP64 and A = u+v-w, B = u-v+w,
+// C = -u+v+w are formed in FP64 before rounding to FP32, because for
+// elongated triangles (two long sides of nearly equal length) the difference
+// cancels and FP32 inputs would leave only a few correct bits; everything
+// after that is FP32.
+__device__ __forceinline__ void at_coeffs_mixed(double u64, double v64, double w64, float nu, float K, float &eu,
+                                                float &ev, float &ew, float &e, bool need_w)
+{
+    const float u = (float)u64, v = (float)v64, w = (float)w64;
+    const float A = (float)((u64 + v64) - w64);
+    const float B = (float)((u64 - v64) + w64);
+    const float C = (float)((v64 - u64) + w64);
+    const float uv = u * v;
+    const float s = rsqrt_f32(uv * w);
+    const float s2 = s * s;
+    const float X = B * C, Y = A * C, Z = A * B;
+    const float Q = (A * X) * s2;
+    const float R = __builtin_fmaf(-2.5f * K, Q, -1.5f * nu);
+    const float S = (X + Y) + Z;
+    const float KS = K * S;
+    const float m2K = -2.0f * K;
+    const float c2s5 = 2.0f * (s2 * s2 * s);
+    eu = c2s5 * __builtin_fmaf(R, v * w, __builtin_fmaf(m2K, Z, KS));
+    ev = c2s5 * __builtin_fmaf(R, u * w, __builtin_fmaf(m2K, Y, KS));
+    ew = need_w ? c2s5 * __builtin_fmaf(R, uv, __builtin_fmaf(m2K, X, KS)) : 0.0f;
+    e = (s2 * s) * __builtin_fmaf(K, Q, nu);
+}
+
+// Dispatch on the per-triplet type: FP64 throughout, or the mixed form.
+template <class T, bool NEED_W>
+__device__ __forceinline__ void at_coeffs_from64(double u, double v, double w, T nu, T K, T &eu, T &ev, T &ew, T &e);
+
 // Gradient coefficients of one triplet in the squared-distance form.
 // Returns eu = 2 dE/du, ev = 2 dE/dv, ew = 2 dE/dw and the energy e, so that
 //   dE/dxi = -(eu dij + ev dik), dE/dxj = eu dij - ew djk, dE/dxk = ev dik + ew djk
@@ -197,6 +229,13 @@ __device__ __forceinline__ void at_coeffs(T u, T v, T w, T nu, T K, T &eu, T &ev
     if constexpr (NEED_W) ew = c2s5 * fma_t<T>(R, uv, fma_t<T>(m2K, X, KS));
     else ew = T(0);
     e = (s2 * s) * fma_t<T>(K, Q, nu);
+}
+
+template <class T, bool NEED_W>
+__device__ __forceinline__ void at_coeffs_from64(double u, double v, double w, T nu, T K, T &eu, T &ev, T &ew, T &e)
+{
+    if constexpr (sizeof(T) == 8) at_coeffs<double, NEED_W>(u, v, w, nu, K, eu, ev, ew, e);
+    else at_coeffs_mixed(u, v, w, nu, K, eu, ev, ew, e, NEED_W);
 }
 
 }  // namespace tb3

@@ -756,22 +756,23 @@ tb3_status tb3_get_profile(const tb3_ctx *cc, double *out, int64_t n)
 }
 
 tb3_status tb3_selftest_fp64(tb3_ctx *c, int64_t n, const double *a, const double *b, double *sqrt_a,
-                             double *a_div_b)
+                             double *a_div_b, double *rsq_a)
 {
     if (!c) return TB3_EINVAL;
     if (n < 0 || (n > 0 && (!a || !b || !sqrt_a || !a_div_b))) return fail(c, TB3_EINVAL, "bad arguments");
     return guarded(c, [&] {
         if (n == 0) return;
         double *buf = nullptr;
-        HIPCHK(hipMalloc((void **)&buf, (size_t)n * 4 * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&buf, (size_t)n * 5 * sizeof(double)));
         try {
             const size_t bytes = (size_t)n * sizeof(double);
             HIPCHK(hipMemcpyAsync(buf, a, bytes, hipMemcpyHostToDevice, c->stream));
             HIPCHK(hipMemcpyAsync(buf + n, b, bytes, hipMemcpyHostToDevice, c->stream));
-            launch_selftest_fp64(c->stream, n, buf, buf + n, buf + 2 * n, buf + 3 * n);
+            launch_selftest_fp64(c->stream, n, buf, buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(sqrt_a, buf + 2 * n, bytes, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipMemcpyAsync(a_div_b, buf + 3 * n, bytes, hipMemcpyDeviceToHost, c->stream));
+            if (rsq_a) HIPCHK(hipMemcpyAsync(rsq_a, buf + 4 * n, bytes, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
         } catch (...) {
             (void)hipFree(buf);

@@ -531,18 +531,11 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
             const double djx = xj - xp, djy = yj - yp, djz = zj - zp;
             const double dkx = xk - xp, dky = yk - yp, dkz = zk - zp;
             const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
-            T u, v, w;
-            if constexpr (sizeof(T) == 8) {
-                u = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
-                v = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
-                w = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-            } else {
-                u = (T)__builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
-                v = (T)__builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
-                w = (T)__builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-            }
+            const double u = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+            const double v = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+            const double w = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
             T eu, ev, ew, et;
-            at_coeffs<T, GRAD>(u, v, w, nu, K, eu, ev, ew, et);
+            at_coeffs_from64<T, GRAD>(u, v, w, nu, K, eu, ev, ew, et);
             if (GRAD) {
                 const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
                 gpx = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gpx));
@@ -809,7 +802,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
                     const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
                     const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
                     T eu, ev, ew, et;
-                    at_coeffs<T, GRAD>((T)u64, (T)v64, (T)w64, nu, K, eu, ev, ew, et);
+                    at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
                     if (GRAD) {
                         const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
                         gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
@@ -1034,17 +1027,15 @@ __global__ __launch_bounds__(WAVE) void k_auta_exact(AutaExactDev a)
     if (role == 0 && lane == 0) add_count(a.st, 0, accepted);
 }
 
-// Cutoff algorithm: one wave per (owner, role) collects N(p) from the cells,
-// sorts it by index in LDS (bitonic), then runs that role's stream.  (Three
-// waves per owner rebuild the same list; the parallelism pays for that.)
+// Cutoff algorithm: one wave per owner collects N(p) from the cells, sorts it
+// by index in LDS (bitonic), then runs the three role streams in turn over
+// that one list.
 __global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
 {
     extern __shared__ int sid_ex[];
     __shared__ ExactShared sh;
     __shared__ int sh_start[32], sh_pre[32];
-    // block b: role b % 3 of the owner in slot owner_slot(b / 3)
-    const int role = (int)(blockIdx.x % 3);
-    const int slot = owner_slot((int)(blockIdx.x / 3), a.n);
+    const int slot = owner_slot((int)blockIdx.x, a.n);
     if (slot >= a.n) return;
     const int p = a.cell_idx[slot];
     const int lane = lane_id();
@@ -1098,16 +1089,17 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
     const int split = below;
 
     const LdsList list{sid_ex};
-    double acc = 0.0;
-    unsigned long long accepted = 0;
-    if (role == 0)
-        exact_role<0>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc, accepted);
-    else if (role == 1)
-        exact_role<1>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc, accepted);
-    else
-        exact_role<2>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc, accepted);
-    if (lane < 3) a.facc[(long long)(3 * role + lane) * a.n + p] = acc;
-    if (role == 0 && lane == 0) add_count(a.st, 0, accepted);
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    unsigned long long accepted = 0, unused = 0;
+    exact_role<0>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc0, accepted);
+    exact_role<1>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc1, unused);
+    exact_role<2>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc2, unused);
+    if (lane < 3) {
+        a.facc[(long long)lane * a.n + p] = acc0;
+        a.facc[(long long)(3 + lane) * a.n + p] = acc1;
+        a.facc[(long long)(6 + lane) * a.n + p] = acc2;
+    }
+    if (lane == 0) add_count(a.st, 0, accepted);
 }
 
 // ---------------------------------------------------------------------------
@@ -1190,18 +1182,10 @@ __global__ __launch_bounds__(WAVE) void k_forces_tiles(TileDev a)
             const double xk = kx[kk], yk = ky[kk], zk = kz[kk];
             const double dikx = xk - xi, diky = yk - yi, dikz = zk - zi;
             const double djkx = xk - xj, djky = yk - yj, djkz = zk - zj;
-            T u, v, w;
-            if constexpr (sizeof(T) == 8) {
-                u = u64;
-                v = __builtin_fma(dikx, dikx, __builtin_fma(diky, diky, dikz * dikz));
-                w = __builtin_fma(djkx, djkx, __builtin_fma(djky, djky, djkz * djkz));
-            } else {
-                u = (T)u64;
-                v = (T)__builtin_fma(dikx, dikx, __builtin_fma(diky, diky, dikz * dikz));
-                w = (T)__builtin_fma(djkx, djkx, __builtin_fma(djky, djky, djkz * djkz));
-            }
+            const double v = __builtin_fma(dikx, dikx, __builtin_fma(diky, diky, dikz * dikz));
+            const double w = __builtin_fma(djkx, djkx, __builtin_fma(djky, djky, djkz * djkz));
             T eu, ev, ew, et;
-            at_coeffs<T, true>(u, v, w, nu, K, eu, ev, ew, et);
+            at_coeffs_from64<T, true>(u64, v, w, nu, K, eu, ev, ew, et);
             const double cu = ok ? (double)eu : 0.0;
             const double cv = ok ? (double)ev : 0.0;
             const double cw = ok ? (double)ew : 0.0;
@@ -1260,12 +1244,13 @@ __global__ __launch_bounds__(WAVE) void k_forces_tiles(TileDev a)
 }
 
 __global__ void k_selftest_fp64(long long n, const double *__restrict__ a, const double *__restrict__ b,
-                                double *__restrict__ sq, double *__restrict__ q)
+                                double *__restrict__ sq, double *__restrict__ q, double *__restrict__ rs)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     sq[i] = __builtin_sqrt(a[i]);
     q[i] = a[i] / b[i];
+    rs[i] = __builtin_amdgcn_rsq(a[i]);
 }
 
 GridDev grid_dev(const CellGrid &g)
@@ -1466,7 +1451,7 @@ void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     const CutDev a = cut_dev(d, g, c, nu, cap, st, 0);
     const size_t shm = p3bca_exact_lds(cap);
     allow_lds(k_p3bca_exact, shm);
-    hipLaunchKernelGGL(k_p3bca_exact, dim3(3 * owner_grid(d.n)), dim3(WAVE), shm, s, a);
+    hipLaunchKernelGGL(k_p3bca_exact, dim3(owner_grid(d.n)), dim3(WAVE), shm, s, a);
 }
 
 void launch_auta_exact(hipStream_t s, const DeviceState &d, double nu, Status *st)
@@ -1511,10 +1496,11 @@ void launch_forces_tiles(hipStream_t s, const DeviceState &d, const unsigned lon
     }
 }
 
-void launch_selftest_fp64(hipStream_t s, long long n, const double *a, const double *b, double *sq, double *q)
+void launch_selftest_fp64(hipStream_t s, long long n, const double *a, const double *b, double *sq, double *q,
+                          double *rs)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_selftest_fp64, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, b, sq, q);
+    hipLaunchKernelGGL(k_selftest_fp64, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, b, sq, q, rs);
 }
 
 }  // namespace tb3

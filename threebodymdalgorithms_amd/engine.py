@@ -177,11 +177,12 @@ class Engine:
                 "update_ms": v[4], "triplets": int(v[5])}
 
     def selftest_fp64(self, a, b):
+        """Device sqrt(a), a / b and the raw v_rsq_f64 estimate of a."""
         a = np.ascontiguousarray(a, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)
-        sq, q = np.zeros_like(a), np.zeros_like(a)
-        self._check(self._lib.tb3_selftest_fp64(self._ctx, a.size, _dp(a), _dp(b), _dp(sq), _dp(q)))
-        return sq, q
+        sq, q, rs = np.zeros_like(a), np.zeros_like(a), np.zeros_like(a)
+        self._check(self._lib.tb3_selftest_fp64(self._ctx, a.size, _dp(a), _dp(b), _dp(sq), _dp(q), _dp(rs)))
+        return sq, q, rs
 
 
 # -- particle CSV (utility/utility.cpp:14-24, 54-68) ---------------------------
