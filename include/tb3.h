@@ -107,8 +107,34 @@ tb3_status tb3_step(tb3_ctx *ctx, double dt, const double g[3], uint64_t *n_buff
 
 /* `steps` iterations of tb3_step without host round trips between them
  * (Simulation.cpp:30 loop).  counts (may be NULL) receives `steps` accepted
- * triplet counts. */
+ * triplet counts (summed over the replicas of a batch).  From two steps on,
+ * the steps are replays of one captured HIP graph of a step, with one host
+ * synchronisation per call; a step whose neighbour staging overflows is
+ * finished on the host and the replays resume, with the same results as
+ * tb3_step. */
 tb3_status tb3_run(tb3_ctx *ctx, int steps, double dt, const double g[3], uint64_t *counts);
+
+/* Replica batches: independent particle systems of uneven size swept in the
+ * same launches (the cell kernel's device queue hands out cells of every
+ * replica).  No counterpart in the reference, which runs one system per
+ * process: each replica behaves as its own `mpiexec -n 1 ./main` run of
+ * P3BCA (Simulation.cpp:28-81), with its own periodic domain (the
+ * RegularGridDecomposition.cpp:135-166 rule on its own particles, or the
+ * domain_L of tb3_configure).
+ *
+ * tb3_batch_set splits the particles of tb3_set_particles into nrep
+ * replicas: replica r holds rows offsets[r] .. offsets[r+1]-1
+ * (offsets[0] = 0, offsets[nrep] = n, nondecreasing).  Needs P3BCA with a
+ * positive cutoff (TB3_EUNSUPPORTED otherwise); nrep = 1 restores one system,
+ * and tb3_set_particles always does. */
+tb3_status tb3_batch_set(tb3_ctx *ctx, int32_t nrep, const int64_t *offsets);
+/* tb3_run with per-replica counts: counts (may be NULL) is steps x nrep. */
+tb3_status tb3_batch_run(tb3_ctx *ctx, int steps, double dt, const double g[3], uint64_t *counts);
+/* Per-replica accepted-triplet counts of the last sweep (nrep entries). */
+tb3_status tb3_batch_counts(tb3_ctx *ctx, int32_t nrep, uint64_t *counts);
+/* Per-replica energies E[nrep] of the current positions (the held forces are
+ * untouched); counts (may be NULL) receives the per-replica triplet counts. */
+tb3_status tb3_batch_energy(tb3_ctx *ctx, int32_t nrep, double *E, uint64_t *counts);
 
 /* SimulationStep alone (no predictor or update): reset, sweep and role sum
  * (P3BCA.cpp:516-805 / AUTA.cpp:236-383 at one process).  The force is read
@@ -135,6 +161,16 @@ tb3_status tb3_energy_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, double n
                           uint64_t *n_triplets);
 tb3_status tb3_energy_grad_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, double nu, double *d_E,
                                double *d_dEdx, void *stream, uint64_t *n_triplets);
+
+/* Autotuner.  No counterpart in the reference.  For the cutoff algorithm,
+ * times a force sweep of the current particles (on a copy; the state is
+ * untouched) for each cell width / cutoff ratio in {1, 1.15, 1.3, 1.5, 1.75,
+ * 2} and each count of resident cell-kernel workgroups per CU, keeps the
+ * fastest, and reports it (each output may be NULL): the best sweep time in
+ * ms (HIP events, minimum of `reps` runs after one warm-up), the ratio and
+ * the workgroups per CU (0 when the per-owner kernel is in charge).  The
+ * all-triplet sweeps have nothing to tune and return at once. */
+tb3_status tb3_autotune(tb3_ctx *ctx, int reps, double *best_ms, double *cell_ratio, int *blocks_per_cu);
 
 /* Per-phase timing with HIP events recorded on the context's stream around
  * each phase of tb3_step / tb3_run / tb3_compute_forces.  No counterpart in

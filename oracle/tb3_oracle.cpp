@@ -538,4 +538,120 @@ double orc_forces_fast(int64_t n, const double *pos, double nu, double cutoff, c
     return E;
 }
 
+double orc_sweep_cells_fast(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f,
+                            uint64_t *count, int nthreads)
+{
+    /* A CPU implementation written for speed rather than for the reference's
+     * order: cells of width >= cutoff (27-cell stencil, periodic wrap),
+     * owner = smallest index, the (u,v,w) formulation with Newton's third
+     * law, thread-private force arrays reduced at the end. */
+    nthreads = resolve_threads(nthreads);
+    const double c2 = cutoff * cutoff;
+    const double K = 0.375 * nu;
+    int nc[3];
+    double lo[3];
+    for (int d = 0; d < 3; ++d) {
+        double mn = 1e300;
+        for (int64_t i = 0; i < n; ++i) mn = std::min(mn, pos[3 * i + d]);
+        lo[d] = n > 0 ? mn : 0.0;
+        const double q = std::floor(L[d] / (cutoff * (1.0 + 1e-6)));
+        nc[d] = q >= 3.0 ? (int)std::min(q, 512.0) : 1;
+    }
+    const int64_t ncell = (int64_t)nc[0] * nc[1] * nc[2];
+    std::vector<int> cell_of((size_t)n), start((size_t)ncell + 1, 0), idx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        int cc[3];
+        for (int d = 0; d < 3; ++d) {
+            double t = (pos[3 * i + d] - lo[d]) / L[d];
+            t -= std::floor(t);
+            cc[d] = std::min(nc[d] - 1, std::max(0, (int)(t * nc[d])));
+        }
+        cell_of[(size_t)i] = (cc[2] * nc[1] + cc[1]) * nc[0] + cc[0];
+        start[(size_t)cell_of[(size_t)i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; ++c) start[(size_t)c + 1] += start[(size_t)c];
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int64_t i = 0; i < n; ++i) idx[(size_t)fill[(size_t)cell_of[(size_t)i]]++] = (int)i;
+    }
+    std::vector<std::vector<double>> fpriv((size_t)nthreads, std::vector<double>((size_t)(3 * n), 0.0));
+    double E = 0.0;
+    uint64_t cnt = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : E, cnt)
+    {
+#ifdef _OPENMP
+        std::vector<double> &fp = fpriv[(size_t)omp_get_thread_num()];
+#else
+        std::vector<double> &fp = fpriv[0];
+#endif
+        std::vector<int> l;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t p = 0; p < n; ++p) {
+            const double *P = pos + 3 * p;
+            l.clear();
+            const int c = cell_of[(size_t)p];
+            const int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
+            const int wx = nc[0] >= 3 ? 1 : 0, wy = nc[1] >= 3 ? 1 : 0, wz = nc[2] >= 3 ? 1 : 0;
+            for (int oz = -wz; oz <= wz; ++oz)
+                for (int oy = -wy; oy <= wy; ++oy)
+                    for (int ox = -wx; ox <= wx; ++ox) {
+                        const int xx = (cx + ox + nc[0]) % nc[0], yy = (cy + oy + nc[1]) % nc[1];
+                        const int zz = (cz + oz + nc[2]) % nc[2];
+                        const int cc = (zz * nc[1] + yy) * nc[0] + xx;
+                        for (int t = start[(size_t)cc]; t < start[(size_t)cc + 1]; ++t) {
+                            const int q = idx[(size_t)t];
+                            if (q <= p) continue;
+                            if (sqr_dist_periodic(P[0], P[1], P[2], pos[3 * q], pos[3 * q + 1], pos[3 * q + 2], L) > c2)
+                                continue;
+                            l.push_back(q);
+                        }
+                    }
+            double gp[3] = {0, 0, 0};
+            for (size_t a = 0; a < l.size(); ++a) {
+                const int j = l[a];
+                const double *J = pos + 3 * j;
+                const double dj[3] = {J[0] - P[0], J[1] - P[1], J[2] - P[2]};
+                const double u = dj[0] * dj[0] + dj[1] * dj[1] + dj[2] * dj[2];
+                double gj[3] = {0, 0, 0};
+                for (size_t b = a + 1; b < l.size(); ++b) {
+                    const int k = l[b];
+                    const double *Kp = pos + 3 * k;
+                    if (sqr_dist_periodic(J[0], J[1], J[2], Kp[0], Kp[1], Kp[2], L) > c2) continue;
+                    const double dk[3] = {Kp[0] - P[0], Kp[1] - P[1], Kp[2] - P[2]};
+                    const double e[3] = {Kp[0] - J[0], Kp[1] - J[1], Kp[2] - J[2]};
+                    const double v = dk[0] * dk[0] + dk[1] * dk[1] + dk[2] * dk[2];
+                    const double w = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+                    const double s = 1.0 / std::sqrt(u * v * w);
+                    const double s2 = s * s;
+                    const double A = u + v - w, B = u - v + w, C = -u + v + w;
+                    const double X = B * C, Y = A * C, Z = A * B;
+                    const double Q = A * X * s2;
+                    const double R = -1.5 * nu - 2.5 * K * Q;
+                    const double S = X + Y + Z;
+                    const double c2s5 = 2.0 * s2 * s2 * s;
+                    const double eu = c2s5 * (R * v * w + K * (S - 2.0 * Z));
+                    const double ev = c2s5 * (R * u * w + K * (S - 2.0 * Y));
+                    const double ew = c2s5 * (R * u * v + K * (S - 2.0 * X));
+                    for (int d = 0; d < 3; ++d) {
+                        gp[d] -= eu * dj[d] + ev * dk[d];
+                        gj[d] += eu * dj[d] - ew * e[d];
+                        fp[3 * (size_t)k + d] += ev * dk[d] + ew * e[d];
+                    }
+                    E += s2 * s * (nu + K * Q);
+                    cnt++;
+                }
+                for (int d = 0; d < 3; ++d) fp[3 * (size_t)j + d] += gj[d];
+            }
+            for (int d = 0; d < 3; ++d) fp[3 * (size_t)p + d] += gp[d];
+        }
+    }
+    for (int64_t i = 0; i < 3 * n; ++i) {
+        double a = 0.0;
+        for (int t = 0; t < nthreads; ++t) a += fpriv[(size_t)t][(size_t)i];
+        f[i] = a;
+    }
+    if (count) *count = cnt;
+    return E;
+}
+
 }  // extern "C"

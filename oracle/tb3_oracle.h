@@ -105,6 +105,13 @@ double orc_energy(int64_t n, const double *pos, double nu, double cutoff, const 
 double orc_forces_fast(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f,
                        uint64_t *count, int nthreads);
 
+/* A fast CPU sweep for the cutoff algorithm (the multi-core CPU baseline):
+ * cell list, owner = smallest index, (u,v,w) formulation, Newton's third
+ * law, OpenMP over owners.  Same triplet set as the reference; forces agree
+ * to rounding.  Expects every particle inside [min, min + L) per axis. */
+double orc_sweep_cells_fast(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f,
+                            uint64_t *count, int nthreads);
+
 /* Bounded-sample timing of the reference-shaped brute sweep: runs rows
  * i in [0, rows) of the i<j<k loop only and returns the candidates visited. */
 uint64_t orc_sweep_brute_rows(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f0,

@@ -56,6 +56,9 @@ def lib():
         L.orc_forces_fast.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _u64p,
                                       ctypes.c_int]
         L.orc_forces_fast.restype = ctypes.c_double
+        L.orc_sweep_cells_fast.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _u64p,
+                                           ctypes.c_int]
+        L.orc_sweep_cells_fast.restype = ctypes.c_double
         L.orc_sweep_brute_rows.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
                                            _dp, ctypes.c_int64, _u64p]
         L.orc_sweep_brute_rows.restype = ctypes.c_uint64
@@ -157,4 +160,12 @@ def forces_fast(pos, nu=1.0, cutoff=-1.0, L=None, threads=0):
     cnt = ctypes.c_uint64(0)
     Lp = _p(_c(L)) if L is not None else None
     E = lib().orc_forces_fast(pos.shape[0], _p(pos), nu, cutoff, Lp, _p(f), ctypes.byref(cnt), threads)
+    return f, E, int(cnt.value)
+
+
+def sweep_cells_fast(pos, nu=1.0, cutoff=2.0, L=None, threads=0):
+    pos = _c(pos)
+    f = np.zeros_like(pos)
+    cnt = ctypes.c_uint64(0)
+    E = lib().orc_sweep_cells_fast(pos.shape[0], _p(pos), nu, cutoff, _p(_c(L)), _p(f), ctypes.byref(cnt), threads)
     return f, E, int(cnt.value)

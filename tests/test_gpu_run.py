@@ -1,0 +1,138 @@
+"""Multi-step runs (tb3_run) replay one captured step graph.  Each run is
+checked against the same steps taken one tb3_step at a time, with eager work
+and host synchronisation between the replays:
+  * exact path: every state array and every count bit-identical;
+  * FP64 path: counts identical, state within 1e-12 (its force sums are
+    atomic, so their order is free);
+  * a run in which a sweep overflows part way (two clusters that merge):
+    the halted replays leave the state alone and the host finishes the step,
+    so the result still equals the one-step-at-a-time run bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import threebodymdalgorithms_amd as tb3
+from conftest import uniform_particles
+
+pytestmark = pytest.mark.gpu
+
+G = np.array([0.0, -9.81, 0.0])
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = tb3.Engine(0)
+    yield e
+    e.close()
+
+
+def state_after(eng, P, V, algo, cutoff, precision, schedule, L=None, nu=1.0):
+    """Runs the schedule (list of ("step", 1) / ("run", k)) from (P, V)."""
+    eng.set_particles(P, V, np.zeros_like(P), np.ones(len(P)))
+    eng.configure(algo, nu=nu, cutoff=cutoff, precision=precision, domain_L=L)
+    counts = []
+    for kind, k in schedule:
+        if kind == "step":
+            counts.append(eng.step(0.001, G)[1])
+        else:
+            counts.extend(int(c) for c in eng.run(k, 0.001, G))
+    return eng.get_particles(), counts
+
+
+def assert_same(a, ca, b, cb, exact):
+    assert ca == cb
+    for name in ("pos", "vel", "acc", "f"):
+        if exact:
+            assert np.array_equal(a[name], b[name]), name
+        else:
+            scale = np.abs(b[name]).max() + 1e-300
+            assert np.abs(a[name] - b[name]).max() <= 1e-12 * scale, name
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp64", "fp32"])
+def test_run_equals_steps_p3bca(eng, precision):
+    """Exact: 12 steps bit for bit.  FP64 / FP32: the atomic force sums make
+    the trajectory order-dependent at the last bit and the dense random start
+    amplifies that quickly, so the state is compared after 2 steps and the
+    counts over 12."""
+    n = 3000
+    P = uniform_particles(n, (n / 2.0) ** (1 / 3), 41)
+    V = np.zeros_like(P)
+    exact = precision == "exact"
+    say(f"run vs steps p3bca {precision}")
+    ref, cref = state_after(eng, P, V, "p3bca", 2.0, precision, [("step", 1)] * 12)
+    got, cgot = state_after(eng, P, V, "p3bca", 2.0, precision, [("run", 5), ("step", 1), ("run", 6)])
+    assert cgot == cref
+    if exact:
+        assert_same(got, cgot, ref, cref, True)
+    # the captured graph is reused by a second run on a new state
+    got2, cgot2 = state_after(eng, P, V, "p3bca", 2.0, precision, [("run", 12)])
+    assert cgot2 == cref
+    if exact:
+        assert_same(got2, cgot2, ref, cref, True)
+    else:
+        ref2, cref2 = state_after(eng, P, V, "p3bca", 2.0, precision, [("step", 1)] * 2)
+        got3, cgot3 = state_after(eng, P, V, "p3bca", 2.0, precision, [("run", 2)])
+        assert_same(got3, cgot3, ref2, cref2, False)
+
+
+def test_run_equals_steps_auta(eng):
+    P = uniform_particles(300, 5.0, 43)
+    V = np.zeros_like(P)
+    say("run vs steps auta exact")
+    ref, cref = state_after(eng, P, V, "auta", 0.0, "exact", [("step", 1)] * 4)
+    got, cgot = state_after(eng, P, V, "auta", 0.0, "exact", [("run", 4)])
+    assert_same(got, cgot, ref, cref, True)
+
+
+def merging_clusters():
+    rng = np.random.default_rng(8)
+    a = rng.normal(0.0, 0.25, (110, 3)) + np.array([6.0, 10.0, 10.0])
+    b = rng.normal(0.0, 0.25, (110, 3)) + np.array([14.0, 10.0, 10.0])
+    P = np.concatenate([a, b])
+    V = np.zeros_like(P)
+    V[:110, 0] = 600.0   # 0.6 per step at dt = 0.001: the clusters overlap at step 5
+    V[110:, 0] = -600.0
+    return P, V
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp64"])
+def test_run_with_overflow_midway(precision):
+    """nu = 1e-9 keeps the clusters intact while they drift into each other;
+    each schedule starts on a new engine so both meet the overflow."""
+    P, V = merging_clusters()
+    L = np.array([20.0, 20.0, 20.0])
+    say(f"run with a mid-run overflow {precision}")
+    with tb3.Engine(0) as e1:
+        ref, cref = state_after(e1, P, V, "p3bca", 2.0, precision, [("step", 1)] * 9, L=L, nu=1e-9)
+    with tb3.Engine(0) as e2:
+        got, cgot = state_after(e2, P, V, "p3bca", 2.0, precision, [("run", 9)], L=L, nu=1e-9)
+    assert max(cref) > 3 * cref[0]  # the neighbourhoods did grow
+    assert_same(got, cgot, ref, cref, precision == "exact")
+
+
+def test_autotune_keeps_results(eng):
+    """The tuner only changes the cell width and the persistent grid: the
+    triplet set is the same and the forces agree to rounding."""
+    P = uniform_particles(4000, 12.6, 5)
+    eng.set_particles(P)
+    eng.configure("p3bca", cutoff=2.0, precision="fp64")
+    c0 = eng.compute_forces()
+    f0 = eng.get_particles()["f"]
+    say("autotune p3bca n=4000")
+    t = eng.autotune(reps=1)
+    say(f"  tuned {t}")
+    assert t["cell_ratio"] in (1.0, 1.15, 1.3, 1.5, 1.75, 2.0) and t["sweep_ms"] > 0
+    c1 = eng.compute_forces()
+    f1 = eng.get_particles()["f"]
+    assert c1 == c0
+    den = np.linalg.norm(f0, axis=1)
+    num = np.linalg.norm(f1 - f0, axis=1)
+    assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max() <= 1e-12

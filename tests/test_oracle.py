@@ -174,6 +174,22 @@ def test_fast_formulation_agrees_with_reference(oracle):
         assert E == pytest.approx(Et, rel=1e-12)
 
 
+@pytest.mark.parametrize("n,box", [(2000, 10.0), (500, 6.5)])
+def test_fast_cpu_cell_sweep_matches_reference(oracle, n, box):
+    """The multi-core CPU baseline sweeps the same triplet set as the
+    reference's loop and agrees with its forces to rounding."""
+    P = uniform_particles(n, box, 31 + n)
+    L, _ = oracle.domain_size(P)
+    *_, f, cnt = oracle.sweep(P, cutoff=2.0, L=L, mode="owner")
+    ff, E, cntf = oracle.sweep_cells_fast(P, cutoff=2.0, L=L)
+    Et, _ = oracle.energy(P, cutoff=2.0, L=L)
+    assert cntf == cnt
+    num = np.linalg.norm(ff - f, axis=1)
+    den = np.linalg.norm(f, axis=1)
+    assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max() < 1e-12
+    assert E == pytest.approx(Et, rel=1e-12)
+
+
 # --- integrator (structs.hpp:40-99) and domain (RegularGridDecomposition.cpp) --
 
 def test_integrator_known_answers(oracle):

@@ -48,6 +48,10 @@ SIGNATURES = {
     "tb3_get_particles": (ctypes.c_int, [_ctxp, ctypes.c_int64, _i32p, _dp, _dp, _dp, _dp, _dp]),
     "tb3_step": (ctypes.c_int, [_ctxp, ctypes.c_double, _dp, _u64p, _u64p]),
     "tb3_run": (ctypes.c_int, [_ctxp, ctypes.c_int, ctypes.c_double, _dp, _u64p]),
+    "tb3_batch_set": (ctypes.c_int, [_ctxp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "tb3_batch_run": (ctypes.c_int, [_ctxp, ctypes.c_int, ctypes.c_double, _dp, _u64p]),
+    "tb3_batch_counts": (ctypes.c_int, [_ctxp, ctypes.c_int32, _u64p]),
+    "tb3_batch_energy": (ctypes.c_int, [_ctxp, ctypes.c_int32, _dp, _u64p]),
     "tb3_compute_forces": (ctypes.c_int, [_ctxp, _u64p]),
     "tb3_energy": (ctypes.c_int, [_ctxp, _dp, _u64p]),
     "tb3_energy_grad": (ctypes.c_int, [_ctxp, _dp, _dp, _dp, _u64p]),
@@ -55,6 +59,7 @@ SIGNATURES = {
                                       ctypes.c_void_p, _u64p]),
     "tb3_energy_grad_dev": (ctypes.c_int, [_ctxp, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double,
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _u64p]),
+    "tb3_autotune": (ctypes.c_int, [_ctxp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "tb3_set_profiling": (ctypes.c_int, [_ctxp, ctypes.c_int]),
     "tb3_get_profile": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64]),
     "tb3_selftest_fp64": (ctypes.c_int, [_ctxp, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]),

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -108,6 +109,12 @@ struct tb3_ctx {
     Buffers state;    // the particle system
     Buffers scratch;  // staging and the device-pointer API's working set
 
+    // replica batch (tb3_batch_set): particle offsets and per-replica domains
+    int nrep = 1;
+    std::vector<int64_t> rep_off;
+    std::vector<double> rep_L, rep_lo;  // 3 nrep each
+    int rep_cap = 0;                    // capacity of the device replica tables
+
     // cell grid of the cutoff algorithm
     CellGrid grid;
     int grid_cell_cap = 0, grid_n_cap = 0;
@@ -116,6 +123,12 @@ struct tb3_ctx {
     int list_cap = 128;    // LDS neighbour-list capacity per wave (power of two)
     bool cells_ok = true;  // the cell-per-workgroup kernel staged every neighbourhood so far
     int cells_grid = 1;    // its persistent grid (CUs x resident workgroups per CU)
+    int dense_grid = 1;    // persistent grid of the dense-owner kernel
+    int ncu = 1;           // compute units of the device
+    // tuned by tb3_autotune: cell width / cutoff, resident cell-kernel workgroups per CU
+    double cell_ratio = 1.0;
+    int cells_per_cu = 0;  // 0: as many as fit
+    int *d_owners = nullptr;  // owners the cell kernel hands to the dense-owner kernel (n)
 
     // tile triples for the all-triplets fast sweep
     unsigned long long *triples = nullptr;
@@ -125,6 +138,16 @@ struct tb3_ctx {
     Status *d_status = nullptr;
     Status *h_status = nullptr;  // pinned
     Slots *d_slots = nullptr;
+
+    // the replayed step graph of tb3_run, and what its capture baked in
+    hipGraphExec_t step_exec = nullptr;
+    unsigned long long epoch = 1;  // bumped whenever buffers, grid or parameters change
+    unsigned long long graph_epoch = 0;
+    double graph_dt = 0.0, graph_g[3] = {0, 0, 0};
+    int graph_cap = -1;
+    bool graph_cells_ok = false;
+    unsigned long long *d_counts = nullptr;
+    long long counts_cap = 0;
 
     // phase timing (tb3_set_profiling)
     bool prof = false;
@@ -195,43 +218,118 @@ CutoffSpec cutoff_spec(const tb3_ctx *c)
 // cutoff lies in the same or adjacent cells (mod the grid) even after the
 // rounding of the wrapped coordinates; dimensions with fewer than three cells
 // collapse to one.
-void configure_grid(tb3_ctx *c)
+void grid_shape(const tb3_ctx *c, const double *L, int nc[3])
 {
     long long total = 1;
     for (int k = 0; k < 3; ++k) {
-        int nc = 1;
-        if (uses_cutoff(c) && c->L[k] > 0 && std::isfinite(c->L[k])) {
-            const double q = std::floor(c->L[k] / (c->cutoff * (1.0 + 1e-6)));
-            nc = q >= 3.0 ? (int)std::min(q, 1024.0) : 1;
+        nc[k] = 1;
+        if (uses_cutoff(c) && L[k] > 0 && std::isfinite(L[k])) {
+            const double q = std::floor(L[k] / (c->cutoff * c->cell_ratio * (1.0 + 1e-6)));
+            nc[k] = q >= 3.0 ? (int)std::min(q, 1024.0) : 1;
         }
-        c->grid.nc[k] = nc;
-        total *= nc;
+        total *= nc[k];
     }
     while (total > (1LL << 22)) {  // keep the grid bounded: widen the largest dimension
         int kmax = 0;
         for (int k = 1; k < 3; ++k)
-            if (c->grid.nc[k] > c->grid.nc[kmax]) kmax = k;
-        c->grid.nc[kmax] = std::max(3, c->grid.nc[kmax] / 2);
-        total = (long long)c->grid.nc[0] * c->grid.nc[1] * c->grid.nc[2];
+            if (nc[k] > nc[kmax]) kmax = k;
+        nc[kmax] = std::max(3, nc[kmax] / 2);
+        total = (long long)nc[0] * nc[1] * nc[2];
     }
-    c->grid.ncell = (int)total;
-    for (int k = 0; k < 3; ++k) {
-        c->grid.lo[k] = c->lo[k];
-        c->grid.L[k] = c->L[k];
+}
+
+// Publishes the replica tables (or their absence) to the status block.
+void publish_replicas(tb3_ctx *c)
+{
+    Status fields;
+    std::memset(&fields, 0, sizeof(fields));
+    fields.nrep = c->nrep;
+    fields.rep_count = c->nrep > 1 ? c->grid.rep_count : nullptr;
+    fields.rep_energy = c->nrep > 1 ? c->grid.rep_energy : nullptr;
+    const size_t off = offsetof(Status, nrep);
+    HIPCHK(hipMemcpy(reinterpret_cast<char *>(c->d_status) + off, reinterpret_cast<char *>(&fields) + off,
+                     sizeof(Status) - off, hipMemcpyHostToDevice));
+}
+
+void configure_grid(tb3_ctx *c)
+{
+    CellGrid &g = c->grid;
+    if (c->nrep <= 1) {
+        grid_shape(c, c->L, g.nc);
+        g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
+        for (int k = 0; k < 3; ++k) {
+            g.lo[k] = c->lo[k];
+            g.L[k] = c->L[k];
+        }
+        g.nrep = 1;
+    } else {
+        const int R = c->nrep;
+        std::vector<int> rnc((size_t)3 * R), coff((size_t)R + 1, 0), poff((size_t)R + 1, 0);
+        for (int r = 0; r < R; ++r) {
+            grid_shape(c, &c->rep_L[3 * (size_t)r], &rnc[3 * (size_t)r]);
+            const long long cells = (long long)rnc[3 * r] * rnc[3 * r + 1] * rnc[3 * r + 2];
+            if (coff[r] + cells > (1LL << 24)) throw Fail(TB3_EUNSUPPORTED, "the replicas need too many cells");
+            coff[r + 1] = coff[r] + (int)cells;
+            poff[r + 1] = (int)c->rep_off[r + 1];
+        }
+        if (R > c->rep_cap) {
+            dev_alloc(g.p_off, R + 1);
+            dev_alloc(g.c_off, R + 1);
+            dev_alloc(g.rnc, 3LL * R);
+            dev_alloc(g.rlo, 3LL * R);
+            dev_alloc(g.rL, 3LL * R);
+            dev_alloc(g.rep_count, R);
+            dev_alloc(g.rep_energy, R);
+            c->rep_cap = R;
+        }
+        HIPCHK(hipMemcpy(g.p_off, poff.data(), poff.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g.c_off, coff.data(), coff.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g.rnc, rnc.data(), rnc.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g.rlo, c->rep_lo.data(), c->rep_lo.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g.rL, c->rep_L.data(), c->rep_L.size() * sizeof(double), hipMemcpyHostToDevice));
+        g.ncell = coff[R];
+        for (int k = 0; k < 3; ++k) {
+            g.nc[k] = rnc[k];
+            g.lo[k] = c->rep_lo[k];
+            g.L[k] = c->rep_L[k];
+        }
+        g.nrep = R;
     }
+    publish_replicas(c);
     c->cells_ok = true;  // a new grid gets a new chance at the cell kernel
+    ++c->epoch;
+}
+
+// Per-replica domains: the bbox rule on each replica's particles, or the
+// caller's domain size for every replica.
+void replica_domains(tb3_ctx *c, const double *rows)
+{
+    const int R = c->nrep;
+    c->rep_L.assign((size_t)3 * R, 1.0);
+    c->rep_lo.assign((size_t)3 * R, 0.0);
+    for (int r = 0; r < R; ++r) {
+        const long long a = c->rep_off[r], b = c->rep_off[r + 1];
+        if (c->user_L) {
+            for (int k = 0; k < 3; ++k) c->rep_L[3 * r + k] = c->L[k];
+        } else if (b > a) {
+            domain_from_rows(b - a, rows + 3 * a, &c->rep_L[3 * (size_t)r], &c->rep_lo[3 * (size_t)r]);
+        }
+    }
 }
 
 void ensure_grid_buffers(tb3_ctx *c, int n)
 {
     CellGrid &g = c->grid;
     if (g.ncell > c->grid_cell_cap) {
+        ++c->epoch;
         dev_alloc(g.cell_cnt, g.ncell);
         HIPCHK(hipMemset(g.cell_cnt, 0, (size_t)g.ncell * sizeof(int)));
         dev_alloc(g.cell_start, (long long)g.ncell + 1);
         c->grid_cell_cap = g.ncell;
     }
     if (n > c->grid_n_cap) {
+        ++c->epoch;
+        dev_alloc(c->d_owners, n);
         dev_alloc(g.cell_of, n);
         dev_alloc(g.cell_idx, n);
         dev_alloc(g.xs, n);
@@ -241,6 +339,7 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
     }
     const long long need = scan_tmp_elems(std::max<long long>(n, g.ncell));
     if (need > c->scan_tmp_cap) {
+        ++c->epoch;
         dev_alloc(c->scan_tmp, need);
         c->scan_tmp_cap = need;
     }
@@ -249,6 +348,7 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
 void ensure_triples(tb3_ctx *c, long long n)
 {
     if (c->triples_n == n) return;
+    ++c->epoch;
     const long long nt = (n + 63) / 64;
     std::vector<unsigned long long> t;
     t.reserve((size_t)(nt * (nt + 1) * (nt + 2) / 6));
@@ -296,9 +396,12 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
         }
         if (fast_grad || energy) {
             const int slot = (exact && grad) ? 1 : 0;  // the exact pass owns count[0]
-            if (cut && c->cells_ok)
+            if (cut && c->cells_ok) {
                 launch_p3bca_cells(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
-                                   energy ? 1 : 0, std::min(c->cells_grid, c->grid.ncell));
+                                   energy ? 1 : 0, std::min(c->cells_grid, c->grid.ncell), c->d_owners);
+                launch_p3bca_dense(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
+                                   energy ? 1 : 0, c->d_owners, c->dense_grid);
+            }
             else if (cut)
                 launch_p3bca_fast(s, d, c->grid, cs, c->nu, c->list_cap, mixed, fast_grad ? 1 : 0, c->d_status,
                                   slot, energy ? 1 : 0);
@@ -315,12 +418,13 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
 // Prepares a sweep over d: resets the status block, runs the predictor when
 // asked (fused into the binning pass for the cutoff algorithm), zeroes the
 // force where the sweep accumulates atomically and builds the cell list.
-void launch_prepare(tb3_ctx *c, hipStream_t s, DeviceState &d, bool predict, double dt, bool grad)
+void launch_prepare(tb3_ctx *c, hipStream_t s, DeviceState &d, bool predict, double dt, bool grad,
+                    bool in_graph = false)
 {
     const bool cut = uses_cutoff(c);
     const bool exact = c->precision == TB3_FP64_EXACT;
     const bool zero_f = grad && (!exact || d.n < 3);
-    launch_reset_status(s, c->d_status, 0);
+    launch_reset_status(s, c->d_status, 0, in_graph ? 1 : 0);
     if (c->recording) HIPCHK(hipEventRecord(c->ev[0], s));
     if (cut && d.n >= 3) {
         // the predictor is fused into the binning pass: its time is counted
@@ -369,22 +473,12 @@ void accumulate_profile(tb3_ctx *c, unsigned long long triplets)
     c->prof_acc[5] += (double)triplets;
 }
 
-// One pass over d: optional predictor, sweep, optional update.  Handles the
-// neighbour-list overflow protocol and returns the accepted-triplet count.
-unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool integrate, double dt, const double *g,
-                            bool grad, bool energy, bool record)
+// Completes a pass whose sweep overflowed (the status block is on the host):
+// grows the capacity or switches kernels, then sweeps and updates again.
+void finish_overflowed(tb3_ctx *c, hipStream_t s, DeviceState &d, bool integrate, double dt, const double *g,
+                       bool grad, bool energy)
 {
-    c->recording = record && c->prof;
-    struct Reset {
-        tb3_ctx *c;
-        ~Reset() { c->recording = false; }
-    } reset{c};
     const bool exact = c->precision == TB3_FP64_EXACT;
-    launch_prepare(c, s, d, integrate, dt, grad);
-    launch_sweep(c, s, d, grad, energy);
-    if (integrate) launch_update(s, d, dt * 0.5, g, c->d_status);
-    if (c->recording) HIPCHK(hipEventRecord(c->ev[4], s));
-    read_status(c, s);
     for (int retry = 0; grow_after_overflow(c); ++retry) {
         if (retry >= 8) throw Fail(TB3_EHIP, "the cutoff sweep kept overflowing its LDS staging");
         c->recording = false;
@@ -398,13 +492,136 @@ unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool inte
         if (integrate) launch_update(s, d, dt * 0.5, g, c->d_status);
         read_status(c, s);
     }
+}
+
+// One pass over d: optional predictor, sweep, optional update.  Handles the
+// neighbour-list overflow protocol and returns the accepted-triplet count.
+unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool integrate, double dt, const double *g,
+                            bool grad, bool energy, bool record)
+{
+    c->recording = record && c->prof;
+    struct Reset {
+        tb3_ctx *c;
+        ~Reset() { c->recording = false; }
+    } reset{c};
+    launch_prepare(c, s, d, integrate, dt, grad);
+    launch_sweep(c, s, d, grad, energy);
+    if (integrate) launch_update(s, d, dt * 0.5, g, c->d_status);
+    if (c->recording) HIPCHK(hipEventRecord(c->ev[4], s));
+    read_status(c, s);
+    finish_overflowed(c, s, d, integrate, dt, g, grad, energy);
     if (c->recording) accumulate_profile(c, c->h_status->count[0]);
     return c->h_status->count[0];
+}
+
+// Replays of one captured step graph run `steps` steps of tb3_run with one
+// launch per step and one host synchronisation per run.  A step whose sweep
+// overflows halts the replays that follow (they leave state and flags
+// alone); the host completes that step with the overflow protocol and
+// captures again for the rest.
+bool graph_ok(const tb3_ctx *c)
+{
+    if (c->prof || !uses_cutoff(c)) return !c->prof;
+    // a dynamic LDS size above 64 KB needs a function attribute set at launch
+    if (c->precision == TB3_FP64_EXACT) return p3bca_exact_lds(c->list_cap) <= 64 * 1024;
+    return c->cells_ok || p3bca_fast_lds(c->list_cap) <= 64 * 1024;
+}
+
+void capture_step_graph(tb3_ctx *c, double dt, const double *g)
+{
+    if (c->step_exec && c->graph_epoch == c->epoch && c->graph_dt == dt && c->graph_g[0] == g[0] &&
+        c->graph_g[1] == g[1] && c->graph_g[2] == g[2] && c->graph_cap == c->list_cap &&
+        c->graph_cells_ok == c->cells_ok)
+        return;
+    if (c->step_exec) {
+        (void)hipGraphExecDestroy(c->step_exec);
+        c->step_exec = nullptr;
+    }
+    hipStream_t s = c->stream;
+    DeviceState &d = c->state.d;
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        launch_prepare(c, s, d, true, dt, true, true);
+        launch_sweep(c, s, d, true, false);
+        launch_update(s, d, dt * 0.5, g, c->d_status);
+        launch_record_count(s, c->d_status);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(s, &graph));
+    const hipError_t e = hipGraphInstantiate(&c->step_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        c->step_exec = nullptr;
+        HIPCHK(e);
+    }
+    c->graph_epoch = c->epoch;
+    c->graph_dt = dt;
+    for (int k = 0; k < 3; ++k) c->graph_g[k] = g[k];
+    c->graph_cap = c->list_cap;
+    c->graph_cells_ok = c->cells_ok;
+}
+
+// The last pass's accepted-triplet count, or per-replica counts for a batch.
+void step_counts(tb3_ctx *c, uint64_t *out)
+{
+    if (c->nrep > 1) {
+        HIPCHK(hipMemcpy(out, c->grid.rep_count, (size_t)c->nrep * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+    } else {
+        out[0] = c->h_status->count[0];
+    }
+}
+
+// counts: `steps` totals, or steps x nrep per-replica counts for a batch.
+void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t *counts)
+{
+    hipStream_t s = c->stream;
+    DeviceState &d = c->state.d;
+    const int per = c->nrep > 1 ? c->nrep : 1;
+    if ((long long)steps * per > c->counts_cap) {
+        dev_alloc(c->d_counts, (long long)steps * per);
+        c->counts_cap = (long long)steps * per;
+    }
+    int done = 0;
+    while (done < steps) {
+        if (!graph_ok(c)) {  // e.g. the overflow protocol grew the lists past 64 KB of LDS
+            for (; done < steps; ++done) {
+                run_pass(c, s, d, true, dt, g, true, false, false);
+                if (counts) step_counts(c, counts + (size_t)done * per);
+            }
+            return;
+        }
+        // what a step would allocate is allocated before the capture
+        if (uses_cutoff(c) && d.n >= 3) ensure_grid_buffers(c, d.n);
+        if (!uses_cutoff(c) && d.n >= 3 && c->precision != TB3_FP64_EXACT) ensure_triples(c, d.n);
+        capture_step_graph(c, dt, g);
+        launch_begin_steps(s, c->d_status, c->d_counts);
+        for (int k = done; k < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec, s));
+        read_status(c, s);
+        const int ok = c->h_status->nstep;
+        if (ok < 0 || ok > steps - done) throw Fail(TB3_EHIP, "replayed steps reported a bad step count");
+        if (ok > 0 && counts)
+            HIPCHK(hipMemcpy(counts + (size_t)done * per, c->d_counts, (size_t)ok * per * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+        done += ok;
+        if (done < steps) {
+            // step `done` overflowed: its predictor ran, its update did not
+            if (!c->h_status->flags[2]) throw Fail(TB3_EHIP, "replayed steps halted without an overflow");
+            finish_overflowed(c, s, d, true, dt, g, true, false);
+            if (counts) step_counts(c, counts + (size_t)done * per);
+            ++done;
+        }
+    }
 }
 
 void recompute_domain(tb3_ctx *c)
 {
     if (c->user_L) {
+        if (c->nrep > 1) replica_domains(c, nullptr);
         c->have_L = true;
         return;
     }
@@ -416,10 +633,38 @@ void recompute_domain(tb3_ctx *c)
         HIPCHK(hipMemcpyAsync(rows.data(), stage, rows.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         domain_from_rows(c->n, rows.data(), c->L, c->lo);
+        if (c->nrep > 1) replica_domains(c, rows.data());
         c->have_L = true;
     } else {
         c->have_L = false;
     }
+}
+
+// tb3_run / tb3_batch_run.  per_replica: counts are steps x nrep; otherwise
+// one total per step.
+void run_steps(tb3_ctx *c, int steps, double dt, const double *g, uint64_t *counts, bool per_replica)
+{
+    const int per = c->nrep > 1 ? c->nrep : 1;
+    std::vector<uint64_t> tmp;
+    uint64_t *out = counts;
+    if (counts && !per_replica && per > 1) {
+        tmp.assign((size_t)steps * per, 0);
+        out = tmp.data();
+    }
+    if (steps >= 2 && graph_ok(c)) {
+        run_steps_graph(c, steps, dt, g, out);
+    } else {
+        for (int s = 0; s < steps; ++s) {
+            run_pass(c, c->stream, c->state.d, true, dt, g, true, false, true);
+            if (out) step_counts(c, out + (size_t)s * per);
+        }
+    }
+    if (out != counts)
+        for (int s = 0; s < steps; ++s) {
+            uint64_t t = 0;
+            for (int r = 0; r < per; ++r) t += tmp[(size_t)s * per + r];
+            counts[s] = t;
+        }
 }
 
 }  // namespace
@@ -457,7 +702,10 @@ tb3_status tb3_create(int device, tb3_ctx **out)
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         int ncu = 0;
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        c->ncu = std::max(1, ncu);
         c->cells_grid = std::max(1, ncu) * p3bca_cells_blocks_per_cu();
+        p3bca_dense_setup();
+        c->dense_grid = std::max(1, ncu) * p3bca_dense_blocks_per_cu();
         dev_alloc(c->d_slots, 1);
         HIPCHK(hipMemset(c->d_slots, 0, sizeof(Slots)));
         dev_alloc(c->d_status, 1);
@@ -493,6 +741,16 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->triples);
     dev_free(c->d_status);
     dev_free(c->d_slots);
+    dev_free(c->d_counts);
+    dev_free(c->d_owners);
+    dev_free(c->grid.p_off);
+    dev_free(c->grid.c_off);
+    dev_free(c->grid.rnc);
+    dev_free(c->grid.rlo);
+    dev_free(c->grid.rL);
+    dev_free(c->grid.rep_count);
+    dev_free(c->grid.rep_energy);
+    if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
     for (hipEvent_t &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_status) (void)hipHostFree(c->h_status);
@@ -533,6 +791,7 @@ tb3_status tb3_configure(tb3_ctx *c, int algo, double nu, double cutoff, const d
         }
         recompute_domain(c);
         configure_grid(c);
+        ++c->epoch;
     });
 }
 
@@ -544,6 +803,7 @@ tb3_status tb3_set_particles(tb3_ctx *c, int64_t n, const int32_t *id, const dou
     if (n > 0 && (!pos || !vel || !acc || !mass)) return fail(c, TB3_EINVAL, "pos, vel, acc and mass are required");
     return guarded(c, [&] {
         c->n = n;
+        c->nrep = 1;  // a new particle set is one system until tb3_batch_set splits it
         c->ids.resize((size_t)n);
         for (int64_t i = 0; i < n; ++i) c->ids[(size_t)i] = id ? id[i] : (int32_t)i;
         c->state.ensure(n);
@@ -604,6 +864,8 @@ static tb3_status check_ready(tb3_ctx *c)
 {
     if (uses_cutoff(c) && !c->have_L)
         return fail(c, TB3_EINVAL, "P3BCA with a cutoff needs particles or an explicit domain size");
+    if (c->nrep > 1 && !uses_cutoff(c))
+        return fail(c, TB3_EUNSUPPORTED, "replica batches run P3BCA with a positive cutoff only");
     return TB3_OK;
 }
 
@@ -624,11 +886,65 @@ tb3_status tb3_run(tb3_ctx *c, int steps, double dt, const double g[3], uint64_t
     if (!c || !g || steps < 0) return fail(c, TB3_EINVAL, "bad arguments");
     tb3_status st = check_ready(c);
     if (st != TB3_OK) return st;
+    return guarded(c, [&] { run_steps(c, steps, dt, g, counts, false); });
+}
+
+tb3_status tb3_batch_run(tb3_ctx *c, int steps, double dt, const double g[3], uint64_t *counts)
+{
+    if (!c || !g || steps < 0) return fail(c, TB3_EINVAL, "bad arguments");
+    tb3_status st = check_ready(c);
+    if (st != TB3_OK) return st;
+    return guarded(c, [&] { run_steps(c, steps, dt, g, counts, true); });
+}
+
+tb3_status tb3_batch_set(tb3_ctx *c, int32_t nrep, const int64_t *offsets)
+{
+    if (!c) return TB3_EINVAL;
+    if (nrep < 1 || !offsets) return fail(c, TB3_EINVAL, "nrep >= 1 and offsets are required");
+    if (offsets[0] != 0 || offsets[nrep] != c->n)
+        return fail(c, TB3_EINVAL, "offsets must run from 0 to the particle count");
+    for (int32_t r = 0; r < nrep; ++r)
+        if (offsets[r + 1] < offsets[r]) return fail(c, TB3_EINVAL, "offsets must not decrease");
+    if (nrep > 1 && !uses_cutoff(c))
+        return fail(c, TB3_EUNSUPPORTED, "replica batches run P3BCA with a positive cutoff: configure it first");
     return guarded(c, [&] {
-        for (int s = 0; s < steps; ++s) {
-            const unsigned long long cnt = run_pass(c, c->stream, c->state.d, true, dt, g, true, false, true);
-            if (counts) counts[s] = cnt;
+        c->nrep = nrep;
+        c->rep_off.assign(offsets, offsets + nrep + 1);
+        recompute_domain(c);
+        configure_grid(c);
+    });
+}
+
+tb3_status tb3_batch_counts(tb3_ctx *c, int32_t nrep, uint64_t *counts)
+{
+    if (!c || !counts) return TB3_EINVAL;
+    if (nrep != c->nrep) return fail(c, TB3_EINVAL, "nrep does not match the batch");
+    return guarded(c, [&] { step_counts(c, counts); });
+}
+
+tb3_status tb3_batch_energy(tb3_ctx *c, int32_t nrep, double *E, uint64_t *counts)
+{
+    if (!c || !E) return TB3_EINVAL;
+    if (nrep != c->nrep) return fail(c, TB3_EINVAL, "nrep does not match the batch");
+    tb3_status st = check_ready(c);
+    if (st != TB3_OK) return st;
+    return guarded(c, [&] {
+        c->scratch.ensure(c->n);
+        DeviceState sd = c->scratch.d;
+        const DeviceState &d = c->state.d;
+        if (c->n > 0) {
+            const size_t b = (size_t)c->n * sizeof(double);
+            HIPCHK(hipMemcpyAsync(sd.x, d.x, b, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(sd.y, d.y, b, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(sd.z, d.z, b, hipMemcpyDeviceToDevice, c->stream));
         }
+        run_pass(c, c->stream, sd, false, 0.0, nullptr, false, true, false);
+        if (nrep > 1) {
+            HIPCHK(hipMemcpy(E, c->grid.rep_energy, (size_t)nrep * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            E[0] = c->h_status->energy;
+        }
+        if (counts) step_counts(c, counts);
     });
 }
 
@@ -694,6 +1010,8 @@ static tb3_status dev_common(tb3_ctx *c, int64_t n, const double *d_pos, double 
     if (n < 0 || (n > 0 && !d_pos) || !d_E || (grad && n > 0 && !d_dEdx))
         return fail(c, TB3_EINVAL, "bad device pointers");
     if (!std::isfinite(nu)) return fail(c, TB3_EINVAL, "nu must be finite");
+    if (c->nrep > 1 && n != c->n)
+        return fail(c, TB3_EINVAL, "a replica batch takes positions for all of its particles");
     return guarded(c, [&] {
         hipStream_t s = stream ? (hipStream_t)stream : c->stream;
         c->scratch.ensure(n);
@@ -734,6 +1052,82 @@ tb3_status tb3_energy_grad_dev(tb3_ctx *c, int64_t n, const double *d_pos, doubl
                                double *d_dEdx, void *stream, uint64_t *n_triplets)
 {
     return dev_common(c, n, d_pos, nu, d_E, d_dEdx, stream, n_triplets, true);
+}
+
+tb3_status tb3_autotune(tb3_ctx *c, int reps, double *best_ms, double *cell_ratio, int *blocks_per_cu)
+{
+    if (!c) return TB3_EINVAL;
+    tb3_status st = check_ready(c);
+    if (st != TB3_OK) return st;
+    if (!uses_cutoff(c)) {
+        // the all-triplet sweeps have no tunable geometry
+        if (best_ms) *best_ms = 0.0;
+        if (cell_ratio) *cell_ratio = c->cell_ratio;
+        if (blocks_per_cu) *blocks_per_cu = c->cells_per_cu;
+        return TB3_OK;
+    }
+    if (reps < 1) reps = 3;
+    return guarded(c, [&] {
+        hipStream_t s = c->stream;
+        c->scratch.ensure(c->n);
+        DeviceState sd = c->scratch.d;
+        const DeviceState &d = c->state.d;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        struct Events {
+            hipEvent_t a, b;
+            ~Events()
+            {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+        } guard{e0, e1};
+        const int max_per_cu = p3bca_cells_blocks_per_cu();
+        const double ratios[] = {1.0, 1.15, 1.3, 1.5, 1.75, 2.0};
+        double best = 1e300, best_ratio = c->cell_ratio;
+        int best_blocks = c->cells_per_cu;
+        const bool was_prof = c->prof;
+        c->prof = false;
+        for (double ratio : ratios) {
+            c->cell_ratio = ratio;
+            configure_grid(c);
+            for (int per_cu = max_per_cu; per_cu >= 1; --per_cu) {
+                c->cells_grid = c->ncu * per_cu;
+                double t = 1e300;
+                for (int r = 0; r < reps + 1; ++r) {  // the first run warms up
+                    if (c->n > 0) {
+                        const size_t b = (size_t)c->n * sizeof(double);
+                        HIPCHK(hipMemcpyAsync(sd.x, d.x, b, hipMemcpyDeviceToDevice, s));
+                        HIPCHK(hipMemcpyAsync(sd.y, d.y, b, hipMemcpyDeviceToDevice, s));
+                        HIPCHK(hipMemcpyAsync(sd.z, d.z, b, hipMemcpyDeviceToDevice, s));
+                    }
+                    HIPCHK(hipEventRecord(e0, s));
+                    run_pass(c, s, sd, false, 0.0, nullptr, true, false, false);
+                    HIPCHK(hipEventRecord(e1, s));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    if (r > 0) t = std::min(t, (double)ms);
+                }
+                // a neighbourhood the cell kernel cannot stage leaves the per-owner kernel in charge
+                if (!c->cells_ok) per_cu = 1;
+                if (t < best) {
+                    best = t;
+                    best_ratio = ratio;
+                    best_blocks = c->cells_ok ? per_cu : 0;
+                }
+            }
+        }
+        c->prof = was_prof;
+        c->cell_ratio = best_ratio;
+        c->cells_per_cu = best_blocks;
+        configure_grid(c);
+        c->cells_grid = c->ncu * (best_blocks > 0 ? best_blocks : max_per_cu);
+        if (best_ms) *best_ms = best;
+        if (cell_ratio) *cell_ratio = best_ratio;
+        if (blocks_per_cu) *blocks_per_cu = best_blocks;
+    });
 }
 
 tb3_status tb3_set_profiling(tb3_ctx *c, int enable)

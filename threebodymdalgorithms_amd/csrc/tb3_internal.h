@@ -31,6 +31,18 @@ struct CellGrid {
     int *cell_start = nullptr;  // ncell + 1
     int *cell_idx = nullptr;    // n, particle indices grouped by cell (order within a cell unspecified)
     double *xs = nullptr, *ys = nullptr, *zs = nullptr;  // n, positions in cell_idx order
+
+    // Independent particle systems swept together ("replicas"): replica r
+    // holds particles [p_off[r], p_off[r+1]) and cells [c_off[r], c_off[r+1])
+    // of the global cell numbering, with its own grid shape (rnc) and
+    // periodic domain (rlo, rL).  With nrep == 1 the fields above describe
+    // the one system and the tables are unused.
+    int nrep = 1;
+    int *p_off = nullptr, *c_off = nullptr;    // device, nrep + 1 each
+    int *rnc = nullptr;                        // device, 3 nrep
+    double *rlo = nullptr, *rL = nullptr;      // device, 3 nrep
+    unsigned long long *rep_count = nullptr;   // device, nrep: accepted triplets per replica
+    double *rep_energy = nullptr;              // device, nrep: energy per replica
 };
 
 struct CutoffSpec {
@@ -58,15 +70,37 @@ struct Status {
                                   // kernel skips itself), [3] the cell kernel could not stage a
                                   // neighbourhood (re-run with the per-owner kernel)
     int queue[2];                 // cell queue of the cell kernel; returns to zero after each launch
+    int dqueue[2];                // owner queue of the dense-owner kernel; returns to zero likewise
+    int dense_n;                  // owners the cell kernel handed to the dense-owner kernel
     Slots *slots;                 // partial sums of the sweep kernels
+    // Replayed step graphs (tb3_run): a step whose sweep overflowed sets
+    // halt; every later replay then leaves the state and the flags alone
+    // (no predictor, no update, no status reset), so the host finds the
+    // state just after that step's predictor.  Completed steps append their
+    // count to counts[nstep++].
+    int halt;
+    int nstep;
+    unsigned long long *counts;
+    // replicas (nrep > 1): per-replica counts and energies, cleared with the
+    // status block; a completed replayed step appends nrep counts
+    int nrep;
+    unsigned long long *rep_count;
+    double *rep_energy;
 };
 
 // ---- launchers (all asynchronous on `s`) ----
 void launch_soa_from_rows(hipStream_t s, int n, const double *rows, double *x, double *y, double *z);
 void launch_rows_from_soa(hipStream_t s, int n, const double *x, const double *y, const double *z, double *rows);
-// Clears the status block; keep_nonfinite preserves flags[0].
-void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite);
-// Folds the slot partial sums into st->count / st->energy and clears the slots.
+// Clears the status block (and halt); keep_nonfinite preserves flags[0].
+// in_graph: the reset belongs to a replayed step graph and does nothing
+// while st->halt is set.
+void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite, int in_graph = 0);
+// Starts a run of replayed step graphs: halt = 0, nstep = 0, counts = counts.
+void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts);
+// Appends st->count[0] to st->counts unless the step halted.
+void launch_record_count(hipStream_t s, Status *st);
+// Folds the slot partial sums into st->count / st->energy and clears the
+// slots; sets st->halt when the sweep overflowed (st->flags[2]).
 void launch_finalize(hipStream_t s, Status *st);
 void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h);
 // a = g - f/m, v += a dt/2; skipped entirely when st->flags[2] (overflow) is set.
@@ -102,9 +136,20 @@ void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, 
 // neighbourhood does not fit its LDS staging; the caller then re-runs the
 // sweep with launch_p3bca_fast.
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid);
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid, int *owners);
 // Workgroups of launch_p3bca_cells that fit on one CU at once.
 int p3bca_cells_blocks_per_cu();
+// The cell kernel hands owners whose neighbourhood does not fit its LDS
+// staging to this kernel (one owner per workgroup, lists of up to
+// p3bca_dense_cap() particles); owners beyond that set st->flags[2] and [3]
+// like the cell kernel's own overflow.  `owners` holds n ints.  Call
+// p3bca_dense_setup once per device before the first launch (it sets the
+// kernel's LDS limit, which a captured graph cannot).
+void p3bca_dense_setup();
+int p3bca_dense_cap();
+int p3bca_dense_blocks_per_cu();
+void launch_p3bca_dense(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int grid);
 
 // All-triplet sweeps (NATA / AUTA, or P3BCA without a cutoff).
 void launch_auta_exact(hipStream_t s, const DeviceState &d, double nu, Status *st);

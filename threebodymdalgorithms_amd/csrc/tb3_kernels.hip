@@ -47,22 +47,6 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Unordered pair number t of {0 .. len-1} in folded order: t = d len + r
-// (d < H = (len-1)/2) pairs r with (r + d + 1) mod len; for even len the last
-// len/2 pairs are (r, r + len/2).  d = floor((t + 1/2) / len) in FP64 is exact
-// (the quotient is 1/(2 len) away from an integer, far above the rounding).
-__device__ __forceinline__ void folded_pair(int t, int len, int full, double inv_len, int &ia, int &ib)
-{
-    const int d = (int)(((double)t + 0.5) * inv_len);
-    const int r = t - d * len;
-    const int b1 = r + d + 1;
-    const int ib_full = b1 >= len ? b1 - len : b1;
-    const int ia_half = t - full;
-    const bool in_full = t < full;
-    ia = in_full ? r : ia_half;
-    ib = in_full ? ib_full : ia_half + (len >> 1);
-}
-
 __device__ __forceinline__ void lds_add(double *p, double v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -110,14 +94,42 @@ __global__ void k_rows_from_soa(int n, const double *__restrict__ x, const doubl
     rows[3 * (long long)p + 2] = z[p];
 }
 
-__global__ void k_reset_status(Status *st, int keep_nonfinite)
+__global__ void k_reset_status(Status *st, int keep_nonfinite, int in_graph)
 {
     if (threadIdx.x == 0) {
+        if (in_graph && st->halt) return;
+        st->halt = 0;
+        for (int r = 0; r < st->nrep && st->nrep > 1; ++r) {
+            st->rep_count[r] = 0;
+            st->rep_energy[r] = 0.0;
+        }
         st->count[0] = 0;
         st->count[1] = 0;
         st->energy = 0.0;
+        st->dense_n = 0;
         if (!keep_nonfinite) st->flags[0] = 0;
         st->flags[1] = st->flags[2] = st->flags[3] = 0;
+    }
+}
+
+__global__ void k_begin_steps(Status *st, unsigned long long *counts)
+{
+    if (threadIdx.x == 0) {
+        st->halt = 0;
+        st->nstep = 0;
+        st->counts = counts;
+    }
+}
+
+__global__ void k_record_count(Status *st)
+{
+    if (threadIdx.x == 0 && !st->halt) {
+        if (st->nrep > 1) {
+            for (int r = 0; r < st->nrep; ++r) st->counts[(long long)st->nstep * st->nrep + r] = st->rep_count[r];
+        } else {
+            st->counts[st->nstep] = st->count[0];
+        }
+        st->nstep = st->nstep + 1;
     }
 }
 
@@ -148,6 +160,7 @@ __global__ __launch_bounds__(NSLOT) void k_finalize(Status *st)
             st->count[1] += c1[w];
             st->energy += e0[w];
         }
+        if (st->flags[2]) st->halt = 1;
     }
 }
 
@@ -276,7 +289,62 @@ void scan_rec(hipStream_t s, const TIn *in, long long n_in, TOut *out, long long
 struct GridDev {
     int nc[3];
     double lo[3], L[3];
+    // replicas (nrep > 1): particle and cell ranges, grid shapes and domains
+    int nrep;
+    const int *p_off, *c_off, *rnc;
+    const double *rlo, *rL;
 };
+
+// One replica's grid, resolved for a particle or a cell.
+struct SysGrid {
+    int r, cbase;
+    int nc[3];
+    double lo[3], L[3];
+};
+
+// Last r with off[r] <= v (off ascending, off[0] = 0 <= v).
+__device__ __forceinline__ int find_range(const int *off, int nrep, int v)
+{
+    int lo = 0, hi = nrep - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ SysGrid sys_grid(const GridDev &g, int r)
+{
+    SysGrid s;
+    s.r = r;
+    if (g.nrep <= 1) {
+        s.cbase = 0;
+        for (int k = 0; k < 3; ++k) {
+            s.nc[k] = g.nc[k];
+            s.lo[k] = g.lo[k];
+            s.L[k] = g.L[k];
+        }
+    } else {
+        s.cbase = g.c_off[r];
+        for (int k = 0; k < 3; ++k) {
+            s.nc[k] = g.rnc[3 * r + k];
+            s.lo[k] = g.rlo[3 * r + k];
+            s.L[k] = g.rL[3 * r + k];
+        }
+    }
+    return s;
+}
+
+__device__ __forceinline__ SysGrid grid_of_particle(const GridDev &g, int p)
+{
+    return sys_grid(g, g.nrep <= 1 ? 0 : find_range(g.p_off, g.nrep, p));
+}
+
+__device__ __forceinline__ SysGrid grid_of_cell(const GridDev &g, int c)
+{
+    return sys_grid(g, g.nrep <= 1 ? 0 : find_range(g.c_off, g.nrep, c));
+}
 
 __device__ __forceinline__ int cell_coord(double x, double lo, double L, int nc)
 {
@@ -293,7 +361,7 @@ __device__ __forceinline__ bool finite3(double x, double y, double z)
 }
 
 // Wrapped cell of a position; non-finite positions go to cell 0.
-__device__ __forceinline__ void cell_of_pos(const GridDev &g, double x, double y, double z, int &cx, int &cy, int &cz)
+__device__ __forceinline__ void cell_of_pos(const SysGrid &g, double x, double y, double z, int &cx, int &cy, int &cz)
 {
     if (finite3(x, y, z)) {
         cx = cell_coord(x, g.lo[0], g.L[0], g.nc[0]);
@@ -309,7 +377,7 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.n) return;
-    if (predict) predict_one(d, p, dt, h);
+    if (predict && !st->halt) predict_one(d, p, dt, h);
     if (zero_f) {
         d.fx[p] = 0.0;
         d.fy[p] = 0.0;
@@ -317,9 +385,10 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
     }
     const double px = d.x[p], py = d.y[p], pz = d.z[p];
     if (!finite3(px, py, pz)) atomicOr(&st->flags[0], 1);
+    const SysGrid sg = grid_of_particle(g, p);
     int cx, cy, cz;
-    cell_of_pos(g, px, py, pz, cx, cy, cz);
-    const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    cell_of_pos(sg, px, py, pz, cx, cy, cz);
+    const int c = sg.cbase + (cz * sg.nc[1] + cy) * sg.nc[0] + cx;
     cell_of[p] = c;
     atomicAdd(&cell_cnt[c], 1);
 }
@@ -351,7 +420,7 @@ struct CutDev {
     const double *x, *y, *z;
     GridDev g;
     const int *cell_start, *cell_idx;
-    double c2, Lx, Ly, Lz;
+    double c2;
     double nu;
     int cap;
     double *fx, *fy, *fz;
@@ -359,7 +428,17 @@ struct CutDev {
     Status *st;
     int count_slot;
     int want_energy;
+    unsigned long long *rep_count;  // nrep > 1: per-replica counts (count_slot 0 only)
+    double *rep_energy;
 };
+
+// Per-replica partial sums (replica batches only).
+__device__ __forceinline__ void add_rep(const CutDev &a, int r, unsigned long long c, double e)
+{
+    if (a.g.nrep <= 1) return;
+    if (c && a.count_slot == 0) atomicAdd(&a.rep_count[r], c);
+    if (a.want_energy && e != 0.0) atomicAdd(&a.rep_energy[r], e);
+}
 
 // The candidate set of owner p: the particles of the 27 (or fewer) cells
 // around p's cell, or every particle when some position is not finite (the
@@ -370,22 +449,25 @@ struct Candidates {
     int total;
     int ncells;
     bool brute;
+    int base;  // brute: first particle of the owner's replica
 };
 
-__device__ __forceinline__ Candidates gather_candidates(const CutDev &a, double px, double py, double pz,
-                                                        int *sh_start, int *sh_pre)
+__device__ __forceinline__ Candidates gather_candidates(const CutDev &a, const SysGrid &sg, double px, double py,
+                                                        double pz, int *sh_start, int *sh_pre)
 {
     Candidates c;
     c.brute = a.st->flags[0] != 0;
     if (c.brute) {
-        c.total = a.n;
+        c.base = a.g.nrep <= 1 ? 0 : a.g.p_off[sg.r];
+        c.total = (a.g.nrep <= 1 ? a.n : a.g.p_off[sg.r + 1]) - c.base;
         c.ncells = 0;
         return c;
     }
+    c.base = 0;
     const int lane = lane_id();
     int cx, cy, cz;
-    cell_of_pos(a.g, px, py, pz, cx, cy, cz);
-    const int ncx = a.g.nc[0], ncy = a.g.nc[1], ncz = a.g.nc[2];
+    cell_of_pos(sg, px, py, pz, cx, cy, cz);
+    const int ncx = sg.nc[0], ncy = sg.nc[1], ncz = sg.nc[2];
     const int wx = ncx >= 3 ? 3 : 1, wy = ncy >= 3 ? 3 : 1, wz = ncz >= 3 ? 3 : 1;
     c.ncells = wx * wy * wz;
     int cnt = 0, st = 0;
@@ -395,7 +477,7 @@ __device__ __forceinline__ Candidates gather_candidates(const CutDev &a, double 
         xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
         yy = yy < 0 ? yy + ncy : (yy >= ncy ? yy - ncy : yy);
         zz = zz < 0 ? zz + ncz : (zz >= ncz ? zz - ncz : zz);
-        const int cc = (zz * ncy + yy) * ncx + xx;
+        const int cc = sg.cbase + (zz * ncy + yy) * ncx + xx;
         st = a.cell_start[cc];
         cnt = a.cell_start[cc + 1] - st;
     }
@@ -417,7 +499,7 @@ __device__ __forceinline__ Candidates gather_candidates(const CutDev &a, double 
 __device__ __forceinline__ int candidate(const CutDev &a, const Candidates &c, const int *sh_start,
                                          const int *sh_pre, int t)
 {
-    if (c.brute) return t;
+    if (c.brute) return c.base + t;
     int lo = 0, hi = c.ncells - 1;  // last cell k with sh_pre[k] <= t
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -427,11 +509,11 @@ __device__ __forceinline__ int candidate(const CutDev &a, const Candidates &c, c
     return a.cell_idx[sh_start[lo] + (t - sh_pre[lo])];
 }
 
-__device__ __forceinline__ bool within_cutoff(const CutDev &a, double ax, double ay, double az, double bx, double by,
-                                              double bz)
+__device__ __forceinline__ bool within_cutoff(const CutDev &a, const SysGrid &sg, double ax, double ay, double az,
+                                              double bx, double by, double bz)
 {
     // negation of the rejection test of Algorithm.cpp:95-97
-    return !(sqr_dist_periodic(ax, ay, az, bx, by, bz, a.Lx, a.Ly, a.Lz) > a.c2);
+    return !(sqr_dist_periodic(ax, ay, az, bx, by, bz, sg.L[0], sg.L[1], sg.L[2]) > a.c2);
 }
 
 // Owner slot for block b: blocks are spread round-robin over the 8 XCDs, so
@@ -476,9 +558,10 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
     const int p = a.cell_idx[slot];
     const int lane = lane_id();
     const double xp = a.x[p], yp = a.y[p], zp = a.z[p];
+    const SysGrid sg = grid_of_particle(a.g, p);
 
     // 1. neighbour list N+(p)
-    const Candidates cand = gather_candidates(a, xp, yp, zp, sh_start, sh_pre);
+    const Candidates cand = gather_candidates(a, sg, xp, yp, zp, sh_start, sh_pre);
     int len = 0;
     for (int t0 = 0; t0 < cand.total; t0 += WAVE) {
         const int t = t0 + lane;
@@ -492,7 +575,7 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
                 qx = a.x[q];
                 qy = a.y[q];
                 qz = a.z[q];
-                ok = within_cutoff(a, xp, yp, zp, qx, qy, qz);
+                ok = within_cutoff(a, sg, xp, yp, zp, qx, qy, qz);
             }
         }
         const unsigned long long m = __ballot(ok);
@@ -584,7 +667,7 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
                 ia = t - full;
                 ib = ia + len / 2;
             }
-            ok = within_cutoff(a, sx[ia], sy[ia], sz[ia], sx[ib], sy[ib], sz[ib]);
+            ok = within_cutoff(a, sg, sx[ia], sy[ia], sz[ia], sx[ib], sy[ib], sz[ib]);
         }
         const unsigned long long m = __ballot(ok);
         if (ok) ring[(head + qn + __popcll(m & __lanemask_lt())) & (RING - 1)] = (unsigned)ia | ((unsigned)ib << 16);
@@ -615,6 +698,7 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
     if (lane == 0) {
         if (a.want_energy) add_energy(a.st, esum);
         add_count(a.st, a.count_slot, c);
+        add_rep(a, sg.r, c, esum);
     }
 }
 
@@ -629,41 +713,115 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
 // those of c's forward cells: every particle forward of an owner in c and
 // within the cutoff of it is in this range.  Each
 // wave then claims owners i of cell c, collects i's candidates t > i within
-// the cutoff (a contiguous LDS range, no search), walks the unordered pairs
-// of that list in folded order, tests the third side and evaluates 64 queued
-// triplets at a time.  All three gradients go to per-particle LDS
-// accumulators (LDS atomics); the workgroup adds them to global memory once
-// per staged particle.  A neighbourhood larger than NBCAP, or an owner list
-// larger than LCAP, raises the overflow flags and the host falls back to the
-// per-owner kernel above.
+// the cutoff (a contiguous LDS range, no search) into a contiguous per-wave
+// list, then walks the list's unordered pairs in folded order, 64 at a time,
+// testing the third side and evaluating each passing triplet in place.  The
+// j and k gradients go to per-wave list accumulators (LDS atomics), folded
+// into the staged particles' accumulators once per owner; the workgroup adds
+// those to global memory once per staged particle.  The owners of a
+// neighbourhood larger than NBCAP, and an owner whose list is longer than
+// LCAP, are queued for the dense-owner kernel below instead.
 
 constexpr int CELL_WAVES = 4;
-constexpr int NBCAP = 512;
-constexpr int LCAP = 256;
+constexpr int NBCAP = 384;
+constexpr int LCAP = 96;
 
 struct CellsDev {
     CutDev c;
     const double *xs, *ys, *zs;
     int ncell;
-    int *queue;  // [0] next cell ticket, [1] finished workgroups (both return to 0)
+    int *queue;   // [0] next cell ticket, [1] finished workgroups (both return to 0)
+    int *owners;  // owners handed to the dense-owner kernel (st->dense_n of them)
 };
 
+// The staged cells of cell c (one wave): c first, then its forward
+// neighbour cells.  Publishes each cell's (start, staged prefix) in cst/cpre,
+// cpre[nk] = total, and info = {own_start, own_cnt, nk, total}.
+//
+// Which neighbour cells are "forward".  With >= 4 cells (or 1) in every
+// dimension, pairwise-adjacent cells lie in one unwrapped 2x2x2 block, so
+// orienting each pair of cells by the lexicographic sign of the stencil
+// offset (dz, dy, dx) is acyclic on every triplet: the 13 positive offsets
+// (the half shell) give each triplet exactly one owner.  With exactly 3 cells
+// in a dimension three cells can wrap into a cycle, so the cell index orders
+// them instead.
+__device__ void stage_cells(const CutDev &A, const SysGrid &sg, int c, int *cst, int *cpre, int *info)
+{
+    const int lane = lane_id();
+    const int ncx = sg.nc[0], ncy = sg.nc[1], ncz = sg.nc[2];
+    const int lc = c - sg.cbase;
+    const int cx = lc % ncx, cy = (lc / ncx) % ncy, cz = lc / (ncx * ncy);
+    const int wx = ncx >= 3 ? 3 : 1, wy = ncy >= 3 ? 3 : 1, wz = ncz >= 3 ? 3 : 1;
+    const int nst = wx * wy * wz;
+    int cc = -1;
+    if (lane < nst) {
+        const int ox = lane % wx, oy = (lane / wx) % wy, oz = lane / (wx * wy);
+        int xx = cx + (wx == 3 ? ox - 1 : 0), yy = cy + (wy == 3 ? oy - 1 : 0);
+        int zz = cz + (wz == 3 ? oz - 1 : 0);
+        xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
+        yy = yy < 0 ? yy + ncy : (yy >= ncy ? yy - ncy : yy);
+        zz = zz < 0 ? zz + ncz : (zz >= ncz ? zz - ncz : zz);
+        cc = sg.cbase + (zz * ncy + yy) * ncx + xx;
+    }
+    const bool lexi = (ncx == 1 || ncx >= 4) && (ncy == 1 || ncy >= 4) && (ncz == 1 || ncz >= 4);
+    const int centre = nst / 2;
+    const bool fwd = cc >= 0 && (lexi ? lane > centre : cc > c);
+    const int own_start = A.cell_start[c];
+    const int own_cnt = A.cell_start[c + 1] - own_start;
+    int st = 0, n_in = 0;
+    if (fwd) {
+        st = A.cell_start[cc];
+        n_in = A.cell_start[cc + 1] - st;
+    }
+    const unsigned long long m = __ballot(fwd);
+    int incl = n_in;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const int nfwd = __popcll(m);
+    if (fwd) {
+        const int k = 1 + lane_prefix(m);
+        cst[k] = st;
+        cpre[k] = own_cnt + incl - n_in;
+    }
+    const int total = own_cnt + __shfl(incl, WAVE - 1);
+    if (lane == 0) {
+        cst[0] = own_start;
+        cpre[0] = 0;
+        cpre[1 + nfwd] = total;
+        info[0] = own_start;
+        info[1] = own_cnt;
+        info[2] = 1 + nfwd;
+        info[3] = total;
+    }
+}
+
+// Global slot (cell order) of staged index t.
+__device__ __forceinline__ int staged_slot(const int *cst, const int *cpre, int nk, int t)
+{
+    int k = 0;
+    while (k + 1 < nk && cpre[k + 1] <= t) ++k;
+    return cst[k] + (t - cpre[k]);
+}
+
 template <class T, bool GRAD>
-__global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
+__global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
 {
     __shared__ double nx[NBCAP], ny[NBCAP], nz[NBCAP];
     __shared__ double gxs[NBCAP], gys[NBCAP], gzs[NBCAP];
     __shared__ int nid[NBCAP];
     __shared__ unsigned short wl[CELL_WAVES][LCAP];
-    __shared__ unsigned ring[CELL_WAVES][RING];
-    __shared__ int cst[32], cpre[33];
-    __shared__ int s_cell, s_total, s_own, s_ncells, s_next;
+    __shared__ double lx[CELL_WAVES][LCAP], ly[CELL_WAVES][LCAP], lz[CELL_WAVES][LCAP];
+    __shared__ double lgx[CELL_WAVES][LCAP], lgy[CELL_WAVES][LCAP], lgz[CELL_WAVES][LCAP];
+    __shared__ int cst[32], cpre[33], sinfo[4];
+    __shared__ int s_cell, s_next;
 
     const CutDev &A = a.c;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid / WAVE;
     const T nu = (T)A.nu, K = (T)(0.375 * A.nu);
-    const double Lx = A.Lx, Ly = A.Ly, Lz = A.Lz, c2 = A.c2;
-    const int ncx = A.g.nc[0], ncy = A.g.nc[1], ncz = A.g.nc[2];
+    const double c2 = A.c2;
     double esum = 0.0;
     unsigned cnt = 0;
     // Non-finite positions pair with everything: leave them to the per-owner
@@ -678,77 +836,25 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
         __syncthreads();
         const int c = s_cell;
         if (c >= a.ncell) break;
+        const SysGrid sg = grid_of_cell(A.g, c);
+        const double Lx = sg.L[0], Ly = sg.L[1], Lz = sg.L[2];
 
-        // 1. the staged cells: c first, then neighbour cells with a larger index
-        if (wv == 0) {
-            const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-            const int wx = ncx >= 3 ? 3 : 1, wy = ncy >= 3 ? 3 : 1, wz = ncz >= 3 ? 3 : 1;
-            const int nst = wx * wy * wz;
-            int cc = -1;
-            if (lane < nst) {
-                const int ox = lane % wx, oy = (lane / wx) % wy, oz = lane / (wx * wy);
-                int xx = cx + (wx == 3 ? ox - 1 : 0), yy = cy + (wy == 3 ? oy - 1 : 0);
-                int zz = cz + (wz == 3 ? oz - 1 : 0);
-                xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
-                yy = yy < 0 ? yy + ncy : (yy >= ncy ? yy - ncy : yy);
-                zz = zz < 0 ? zz + ncz : (zz >= ncz ? zz - ncz : zz);
-                cc = (zz * ncy + yy) * ncx + xx;
-            }
-            // Which neighbour cells are "forward".  With >= 4 cells (or 1) in
-            // every dimension, pairwise-adjacent cells lie in one unwrapped
-            // 2x2x2 block, so orienting each pair of cells by the
-            // lexicographic sign of the stencil offset (dz, dy, dx) is
-            // acyclic on every triplet: the 13 positive offsets (the half
-            // shell) give each triplet exactly one owner.  With exactly 3
-            // cells in a dimension three cells can wrap into a cycle, so the
-            // cell index orders them instead.
-            const bool lexi = (ncx == 1 || ncx >= 4) && (ncy == 1 || ncy >= 4) && (ncz == 1 || ncz >= 4);
-            const int centre = nst / 2;
-            const bool fwd = cc >= 0 && (lexi ? lane > centre : cc > c);
-            const int own_start = A.cell_start[c];
-            const int own_cnt = A.cell_start[c + 1] - own_start;
-            int st = 0, n_in = 0;
-            if (fwd) {
-                st = A.cell_start[cc];
-                n_in = A.cell_start[cc + 1] - st;
-            }
-            const unsigned long long m = __ballot(fwd);
-            int incl = n_in;
-#pragma unroll
-            for (int o = 1; o < WAVE; o <<= 1) {
-                const int y = __shfl_up(incl, o);
-                if (lane >= o) incl += y;
-            }
-            const int nfwd = __popcll(m);
-            if (fwd) {
-                const int k = 1 + lane_prefix(m);
-                cst[k] = st;
-                cpre[k] = own_cnt + incl - n_in;
-            }
-            if (lane == 0) {
-                cst[0] = own_start;
-                cpre[0] = 0;
-                s_own = own_cnt;
-                s_ncells = 1 + nfwd;
-            }
-            const int total = own_cnt + __shfl(incl, WAVE - 1);
-            if (lane == 0) {
-                cpre[1 + nfwd] = total;
-                s_total = total;
-            }
-        }
+        // 1. the staged cells: c first, then its forward neighbour cells
+        if (wv == 0) stage_cells(A, sg, c, cst, cpre, sinfo);
         __syncthreads();
-        const int total = s_total, own = s_own, nk = s_ncells;
+        const int total = sinfo[3], own = sinfo[1], nk = sinfo[2];
         if (total > NBCAP) {
-            overflow = true;
-            break;
+            // hand every owner of c to the dense-owner kernel
+            if (tid == 0) s_next = atomicAdd(&A.st->dense_n, own);
+            __syncthreads();
+            for (int t = tid; t < own; t += CELL_WAVES * WAVE) a.owners[s_next + t] = sinfo[0] + t;
+            __syncthreads();
+            continue;
         }
 
         // 2. stage positions and clear the accumulators
         for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
-            int k = 0;
-            while (k + 1 < nk && cpre[k + 1] <= t) ++k;
-            const int slot = cst[k] + (t - cpre[k]);
+            const int slot = staged_slot(cst, cpre, nk, t);
             nx[t] = a.xs[slot];
             ny[t] = a.ys[slot];
             nz[t] = a.zs[slot];
@@ -762,7 +868,8 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
 
         // 3. owners of cell c, claimed one at a time by the waves
         unsigned short *list = wl[wv];
-        unsigned *rg = ring[wv];
+        double *qx = lx[wv], *qy = ly[wv], *qz = lz[wv];
+        double *hx = lgx[wv], *hy = lgy[wv], *hz = lgz[wv];
         for (;;) {
             int i = 0;
             if (lane == 0) i = atomicAdd(&s_next, 1);
@@ -770,81 +877,100 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
             if (i >= own) break;
             const double xi = nx[i], yi = ny[i], zi = nz[i];
 
+            // the owner's forward candidates within the cutoff, in staging
+            // order, copied to this wave's contiguous list
             int len = 0;
             for (int t0 = i + 1; t0 < total; t0 += WAVE) {
                 const int t = t0 + lane;
-                const bool ok = t < total &&
-                                !(sqr_dist_periodic_min(xi, yi, zi, nx[t], ny[t], nz[t], Lx, Ly, Lz) > c2);
+                double xt = 0.0, yt = 0.0, zt = 0.0;
+                bool ok = t < total;
+                if (ok) {
+                    xt = nx[t];
+                    yt = ny[t];
+                    zt = nz[t];
+                    ok = !(sqr_dist_periodic_min(xi, yi, zi, xt, yt, zt, Lx, Ly, Lz) > c2);
+                }
                 const unsigned long long m = __ballot(ok);
                 const int pos = len + lane_prefix(m);
-                if (ok && pos < LCAP) list[pos] = (unsigned short)t;
+                if (ok && pos < LCAP) {
+                    list[pos] = (unsigned short)t;
+                    qx[pos] = xt;
+                    qy[pos] = yt;
+                    qz[pos] = zt;
+                    hx[pos] = 0.0;
+                    hy[pos] = 0.0;
+                    hz[pos] = 0.0;
+                }
                 len += __popcll(m);
             }
             if (len > LCAP) {
-                overflow = true;
+                // too long for this wave's list: the dense-owner kernel takes i
+                if (lane == 0) a.owners[atomicAdd(&A.st->dense_n, 1)] = sinfo[0] + i;
                 continue;
             }
             wave_sync();
 
+            // Every unordered pair of the list once, in folded order: item
+            // t = (o - 1) len + r pairs r with (r + o) mod len, o = 1 .. H;
+            // for even len the last len / 2 items pair r with r + len / 2.
+            // Lane l takes items l, l + 64, ...; (r, o) advance by
+            // (64 mod len, 64 / len) with a carry.  Each pair is tested and,
+            // if it passes, evaluated in place.
             double gix = 0.0, giy = 0.0, giz = 0.0;
-            int head = 0, qn = 0;
-            auto flush = [&](int nq) {
-                wave_sync();
-                if (lane < nq) {
-                    const unsigned e = rg[(head + lane) & (RING - 1)];
-                    const int j = (int)(e & 0xffffu), k = (int)(e >> 16);
-                    const double xj = nx[j], yj = ny[j], zj = nz[j];
-                    const double xk = nx[k], yk = ny[k], zk = nz[k];
-                    const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
-                    const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
-                    const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
-                    const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
-                    const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
-                    const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-                    T eu, ev, ew, et;
-                    at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
-                    if (GRAD) {
-                        const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
-                        gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
-                        giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
-                        giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
-                        lds_add(&gxs[j], __builtin_fma(cu, djx, -cw * ex));
-                        lds_add(&gys[j], __builtin_fma(cu, djy, -cw * ey));
-                        lds_add(&gzs[j], __builtin_fma(cu, djz, -cw * ez));
-                        lds_add(&gxs[k], __builtin_fma(cv, dkx, cw * ex));
-                        lds_add(&gys[k], __builtin_fma(cv, dky, cw * ey));
-                        lds_add(&gzs[k], __builtin_fma(cv, dkz, cw * ez));
-                    }
-                    esum += (double)et;
-                    ++cnt;
-                }
-                wave_sync();
-                head = (head + nq) & (RING - 1);
-                qn -= nq;
-            };
-
-            const int H = (len - 1) / 2;
-            const int full = H * len;
             const int npairs = len * (len - 1) / 2;
-            const double inv_len = len > 0 ? 1.0 / (double)len : 0.0;
-            for (int t0 = 0; t0 < npairs; t0 += WAVE) {
-                const int t = t0 + lane;
-                bool ok = t < npairs;
-                int j = 0, k = 0;
-                if (ok) {
-                    int ia, ib;
-                    folded_pair(t, len, full, inv_len, ia, ib);
-                    j = list[ia];
-                    k = list[ib];
-                    ok = !(sqr_dist_periodic_min(nx[j], ny[j], nz[j], nx[k], ny[k], nz[k], Lx, Ly, Lz) > c2);
+            if (npairs > 0) {
+                const int q64 = WAVE / len, r64 = WAVE - q64 * len;
+                int o = lane / len;
+                int r = lane - o * len;
+                ++o;
+                for (int t0 = 0; t0 < npairs; t0 += WAVE) {
+                    if (t0 + lane < npairs) {
+                        int kk = r + o;
+                        kk = kk >= len ? kk - len : kk;
+                        const double xj = qx[r], yj = qy[r], zj = qz[r];
+                        const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
+                        if (!(sqr_dist_periodic_min(xj, yj, zj, xk, yk, zk, Lx, Ly, Lz) > c2)) {
+                            const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
+                            const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
+                            const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
+                            const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                            const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                            const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+                            T eu, ev, ew, et;
+                            at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
+                            if (GRAD) {
+                                const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
+                                gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
+                                giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
+                                giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
+                                lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
+                                lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
+                                lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
+                                lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
+                                lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
+                                lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
+                            }
+                            esum += (double)et;
+                            ++cnt;
+                        }
+                    }
+                    r += r64;
+                    o += q64;
+                    if (r >= len) {
+                        r -= len;
+                        ++o;
+                    }
                 }
-                const unsigned long long m = __ballot(ok);
-                if (ok) rg[(head + qn + lane_prefix(m)) & (RING - 1)] = (unsigned)j | ((unsigned)k << 16);
-                qn += __popcll(m);
-                if (qn >= WAVE) flush(WAVE);
             }
-            if (qn > 0) flush(qn);
             if (GRAD) {
+                // the list's gradients and the owner's go to the staged accumulators
+                wave_sync();
+                for (int t = lane; t < len; t += WAVE) {
+                    const int q = list[t];
+                    lds_add(&gxs[q], hx[t]);
+                    lds_add(&gys[q], hy[t]);
+                    lds_add(&gzs[q], hz[t]);
+                }
                 gix = wave_sum(gix);
                 giy = wave_sum(giy);
                 giz = wave_sum(giz);
@@ -854,6 +980,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
                     lds_add(&gzs[i], giz);
                 }
             }
+            wave_sync();
         }
         __syncthreads();
 
@@ -865,6 +992,18 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
                 unsafeAtomicAdd(&A.fy[q], gys[t]);
                 unsafeAtomicAdd(&A.fz[q], gzs[t]);
             }
+        }
+        if (A.g.nrep > 1) {
+            // replicas: this cell's sums belong to its replica
+            const unsigned long long cc = wave_sum_u64(cnt);
+            const double ee = A.want_energy ? wave_sum(esum) : 0.0;
+            if (lane == 0) {
+                if (A.want_energy) add_energy(A.st, ee);
+                add_count(A.st, A.count_slot, cc);
+                add_rep(A, sg.r, cc, ee);
+            }
+            cnt = 0;
+            esum = 0.0;
         }
         __syncthreads();
     }
@@ -887,6 +1026,215 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE) void k_p3bca_cells(CellsDev a)
         if (done == (int)gridDim.x - 1) {
             atomicExch(&a.queue[0], 0);
             atomicExch(&a.queue[1], 0);
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Dense-owner sweep: the owners the cell kernel could not stage.
+//
+// One owner per workgroup of DENSE_WAVES waves, taken from the queue the
+// cell kernel filled.  The owner's list is the same forward set the cell
+// kernel would have used (its own cell's later slots, then the forward
+// neighbour cells, within the cutoff), gathered from the cell-ordered
+// positions into LDS by all waves; the list's unordered pairs are then split
+// over the workgroup's lanes in folded order and evaluated in place as in
+// the cell kernel.  A list longer than DCAP raises the overflow flags (the
+// host then re-runs the sweep with the per-owner kernel).
+
+constexpr int DENSE_WAVES = 8;
+constexpr int DCAP = 1536;
+
+struct DenseDev {
+    CutDev c;
+    const double *xs, *ys, *zs;
+    const int *cell_of;
+    const int *owners;
+    int *queue;  // [0] next owner ticket, [1] finished workgroups (both return to 0)
+};
+
+size_t dense_lds() { return (size_t)DCAP * (6 * sizeof(double) + sizeof(int)); }
+
+template <class T, bool GRAD>
+__global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
+{
+    extern __shared__ double dsm[];
+    double *qx = dsm, *qy = qx + DCAP, *qz = qy + DCAP;
+    double *hx = qz + DCAP, *hy = hx + DCAP, *hz = hy + DCAP;
+    int *qid = reinterpret_cast<int *>(hz + DCAP);
+    __shared__ int cst[32], cpre[33], sinfo[4];
+    __shared__ int s_item, s_len;
+    __shared__ double s_gi[3];
+
+    const CutDev &A = a.c;
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid / WAVE;
+    constexpr int NT = DENSE_WAVES * WAVE;
+    const T nu = (T)A.nu, K = (T)(0.375 * A.nu);
+    const double c2 = A.c2;
+    const int nowners = A.st->dense_n;  // written by the cell kernel before this launch
+    double esum = 0.0;
+    unsigned cnt = 0;
+    bool overflow = false;
+
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&a.queue[0], 1);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= nowners) break;
+        const int slot = a.owners[item];
+        const int p = A.cell_idx[slot];
+        const int c = a.cell_of[p];
+        const SysGrid sg = grid_of_cell(A.g, c);
+        const double Lx = sg.L[0], Ly = sg.L[1], Lz = sg.L[2];
+        if (wv == 0) stage_cells(A, sg, c, cst, cpre, sinfo);
+        if (tid == 0) {
+            s_len = 0;
+            s_gi[0] = s_gi[1] = s_gi[2] = 0.0;
+        }
+        __syncthreads();
+        const int i = slot - sinfo[0], nk = sinfo[2], total = sinfo[3];
+        const double xi = a.xs[slot], yi = a.ys[slot], zi = a.zs[slot];
+
+        // the owner's forward list
+        for (int t0 = i + 1; t0 < total; t0 += NT) {
+            const int t = t0 + tid;
+            bool ok = t < total;
+            double xt = 0.0, yt = 0.0, zt = 0.0;
+            int st = 0;
+            if (ok) {
+                st = staged_slot(cst, cpre, nk, t);
+                xt = a.xs[st];
+                yt = a.ys[st];
+                zt = a.zs[st];
+                ok = !(sqr_dist_periodic_min(xi, yi, zi, xt, yt, zt, Lx, Ly, Lz) > c2);
+            }
+            const unsigned long long m = __ballot(ok);
+            int base = 0;
+            if (lane == 0 && m) base = atomicAdd(&s_len, __popcll(m));
+            base = __shfl(base, 0);
+            const int pos = base + lane_prefix(m);
+            if (ok && pos < DCAP) {
+                qx[pos] = xt;
+                qy[pos] = yt;
+                qz[pos] = zt;
+                hx[pos] = 0.0;
+                hy[pos] = 0.0;
+                hz[pos] = 0.0;
+                qid[pos] = A.cell_idx[st];
+            }
+        }
+        __syncthreads();
+        const int len = s_len;
+        if (len > DCAP) {
+            overflow = true;
+            __syncthreads();
+            continue;
+        }
+
+        // the list's unordered pairs, folded, split over the workgroup
+        double gix = 0.0, giy = 0.0, giz = 0.0;
+        const int npairs = len * (len - 1) / 2;
+        if (npairs > 0) {
+            const int qn = NT / len, rn = NT - qn * len;
+            int o = tid / len;
+            int r = tid - o * len;
+            ++o;
+            for (int t0 = 0; t0 < npairs; t0 += NT) {
+                if (t0 + tid < npairs) {
+                    int kk = r + o;
+                    kk = kk >= len ? kk - len : kk;
+                    const double xj = qx[r], yj = qy[r], zj = qz[r];
+                    const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
+                    if (!(sqr_dist_periodic_min(xj, yj, zj, xk, yk, zk, Lx, Ly, Lz) > c2)) {
+                        const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
+                        const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
+                        const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
+                        const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                        const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                        const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+                        T eu, ev, ew, et;
+                        at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
+                        if (GRAD) {
+                            const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
+                            gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
+                            giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
+                            giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
+                            lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
+                            lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
+                            lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
+                            lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
+                            lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
+                            lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
+                        }
+                        esum += (double)et;
+                        ++cnt;
+                    }
+                }
+                r += rn;
+                o += qn;
+                if (r >= len) {
+                    r -= len;
+                    ++o;
+                }
+            }
+        }
+        if (GRAD) {
+            gix = wave_sum(gix);
+            giy = wave_sum(giy);
+            giz = wave_sum(giz);
+            if (lane == 0) {
+                lds_add(&s_gi[0], gix);
+                lds_add(&s_gi[1], giy);
+                lds_add(&s_gi[2], giz);
+            }
+        }
+        __syncthreads();
+        if (GRAD) {
+            for (int t = tid; t < len; t += NT) {
+                const int q = qid[t];
+                unsafeAtomicAdd(&A.fx[q], hx[t]);
+                unsafeAtomicAdd(&A.fy[q], hy[t]);
+                unsafeAtomicAdd(&A.fz[q], hz[t]);
+            }
+            if (tid == 0) {
+                unsafeAtomicAdd(&A.fx[p], s_gi[0]);
+                unsafeAtomicAdd(&A.fy[p], s_gi[1]);
+                unsafeAtomicAdd(&A.fz[p], s_gi[2]);
+            }
+        }
+        if (A.g.nrep > 1) {
+            const unsigned long long cc = wave_sum_u64(cnt);
+            const double ee = A.want_energy ? wave_sum(esum) : 0.0;
+            if (lane == 0) {
+                if (A.want_energy) add_energy(A.st, ee);
+                add_count(A.st, A.count_slot, cc);
+                add_rep(A, sg.r, cc, ee);
+            }
+            cnt = 0;
+            esum = 0.0;
+        }
+        __syncthreads();
+    }
+
+    if (overflow && tid == 0) {
+        atomicOr(&A.st->flags[2], 1);
+        atomicOr(&A.st->flags[3], 1);
+    }
+    if (A.want_energy) esum = wave_sum(esum);
+    const unsigned long long cw = wave_sum_u64(cnt);
+    if (lane == 0) {
+        if (A.want_energy) add_energy(A.st, esum);
+        add_count(A.st, A.count_slot, cw);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        const int done = atomicAdd(&a.queue[1], 1);
+        if (done == (int)gridDim.x - 1) {
+            atomicExch(&a.queue[0], 0);
+            atomicExch(&a.queue[1], 0);
+            atomicExch(&A.st->dense_n, 0);
         }
     }
 }
@@ -1041,8 +1389,9 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
     const int lane = lane_id();
     const int cap = a.cap;
     const double xp = a.x[p], yp = a.y[p], zp = a.z[p];
+    const SysGrid sg = grid_of_particle(a.g, p);
 
-    const Candidates cand = gather_candidates(a, xp, yp, zp, sh_start, sh_pre);
+    const Candidates cand = gather_candidates(a, sg, xp, yp, zp, sh_start, sh_pre);
     int len = 0;
     for (int t0 = 0; t0 < cand.total; t0 += WAVE) {
         const int t = t0 + lane;
@@ -1050,7 +1399,7 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
         int q = 0;
         if (ok) {
             q = candidate(a, cand, sh_start, sh_pre, t);
-            ok = q != p && within_cutoff(a, xp, yp, zp, a.x[q], a.y[q], a.z[q]);
+            ok = q != p && within_cutoff(a, sg, xp, yp, zp, a.x[q], a.y[q], a.z[q]);
         }
         const unsigned long long m = __ballot(ok);
         const int pos = len + __popcll(m & __lanemask_lt());
@@ -1091,15 +1440,18 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
     const LdsList list{sid_ex};
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     unsigned long long accepted = 0, unused = 0;
-    exact_role<0>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc0, accepted);
-    exact_role<1>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc1, unused);
-    exact_role<2>(a.x, a.y, a.z, p, list, len, split, true, a.c2, a.Lx, a.Ly, a.Lz, a.nu, sh, acc2, unused);
+    exact_role<0>(a.x, a.y, a.z, p, list, len, split, true, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc0, accepted);
+    exact_role<1>(a.x, a.y, a.z, p, list, len, split, true, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc1, unused);
+    exact_role<2>(a.x, a.y, a.z, p, list, len, split, true, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc2, unused);
     if (lane < 3) {
         a.facc[(long long)lane * a.n + p] = acc0;
         a.facc[(long long)(3 + lane) * a.n + p] = acc1;
         a.facc[(long long)(6 + lane) * a.n + p] = acc2;
     }
-    if (lane == 0) add_count(a.st, 0, accepted);
+    if (lane == 0) {
+        add_count(a.st, 0, accepted);
+        add_rep(a, sg.r, accepted, 0.0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1261,6 +1613,12 @@ GridDev grid_dev(const CellGrid &g)
         gd.lo[c] = g.lo[c];
         gd.L[c] = g.L[c];
     }
+    gd.nrep = g.nrep;
+    gd.p_off = g.p_off;
+    gd.c_off = g.c_off;
+    gd.rnc = g.rnc;
+    gd.rlo = g.rlo;
+    gd.rL = g.rL;
     return gd;
 }
 
@@ -1276,9 +1634,6 @@ CutDev cut_dev(const DeviceState &d, const CellGrid &g, const CutoffSpec &c, dou
     a.cell_start = g.cell_start;
     a.cell_idx = g.cell_idx;
     a.c2 = c.c2;
-    a.Lx = c.L[0];
-    a.Ly = c.L[1];
-    a.Lz = c.L[2];
     a.nu = nu;
     a.cap = cap;
     a.fx = d.fx;
@@ -1288,6 +1643,8 @@ CutDev cut_dev(const DeviceState &d, const CellGrid &g, const CutoffSpec &c, dou
     a.st = st;
     a.count_slot = count_slot;
     a.want_energy = 0;
+    a.rep_count = g.rep_count;
+    a.rep_energy = g.rep_energy;
     return a;
 }
 
@@ -1317,10 +1674,17 @@ void launch_rows_from_soa(hipStream_t s, int n, const double *x, const double *y
     hipLaunchKernelGGL(k_rows_from_soa, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, y, z, rows);
 }
 
-void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite)
+void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite, int in_graph)
 {
-    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, s, st, keep_nonfinite);
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, s, st, keep_nonfinite, in_graph);
 }
+
+void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts)
+{
+    hipLaunchKernelGGL(k_begin_steps, dim3(1), dim3(64), 0, s, st, counts);
+}
+
+void launch_record_count(hipStream_t s, Status *st) { hipLaunchKernelGGL(k_record_count, dim3(1), dim3(64), 0, s, st); }
 
 void launch_finalize(hipStream_t s, Status *st) { hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NSLOT), 0, s, st); }
 
@@ -1414,7 +1778,7 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
 }
 
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid)
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid, int *owners)
 {
     if (d.n <= 0) return;
     CellsDev a;
@@ -1425,6 +1789,7 @@ void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     a.zs = g.zs;
     a.ncell = g.ncell;
     a.queue = &st->queue[0];
+    a.owners = owners;
     const dim3 gr((unsigned)(grid < 1 ? 1 : grid)), blk(CELL_WAVES * WAVE);
     if (mixed) {
         if (grad) hipLaunchKernelGGL((k_p3bca_cells<float, true>), gr, blk, 0, s, a);
@@ -1442,6 +1807,50 @@ int p3bca_cells_blocks_per_cu()
                                                      CELL_WAVES * WAVE, 0) != hipSuccess || nb < 1)
         nb = 1;
     return nb;
+}
+
+void p3bca_dense_setup()
+{
+    const size_t shm = dense_lds();
+    allow_lds(k_p3bca_dense<double, true>, shm);
+    allow_lds(k_p3bca_dense<double, false>, shm);
+    allow_lds(k_p3bca_dense<float, true>, shm);
+    allow_lds(k_p3bca_dense<float, false>, shm);
+}
+
+int p3bca_dense_cap() { return DCAP; }
+
+int p3bca_dense_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(k_p3bca_dense<double, true>),
+                                                     DENSE_WAVES * WAVE, dense_lds()) != hipSuccess || nb < 1)
+        nb = 1;
+    return nb;
+}
+
+void launch_p3bca_dense(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int grid)
+{
+    if (d.n <= 0) return;
+    DenseDev a;
+    a.c = cut_dev(d, g, c, nu, 0, st, count_slot);
+    a.c.want_energy = want_energy;
+    a.xs = g.xs;
+    a.ys = g.ys;
+    a.zs = g.zs;
+    a.cell_of = g.cell_of;
+    a.owners = owners;
+    a.queue = &st->dqueue[0];
+    const dim3 gr((unsigned)(grid < 1 ? 1 : grid)), blk(DENSE_WAVES * WAVE);
+    const size_t shm = dense_lds();
+    if (mixed) {
+        if (grad) hipLaunchKernelGGL((k_p3bca_dense<float, true>), gr, blk, shm, s, a);
+        else hipLaunchKernelGGL((k_p3bca_dense<float, false>), gr, blk, shm, s, a);
+    } else {
+        if (grad) hipLaunchKernelGGL((k_p3bca_dense<double, true>), gr, blk, shm, s, a);
+        else hipLaunchKernelGGL((k_p3bca_dense<double, false>), gr, blk, shm, s, a);
+    }
 }
 
 void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,

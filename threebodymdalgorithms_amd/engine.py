@@ -56,6 +56,7 @@ class Engine:
         self.nu = 1.0
         self.cutoff = 0.0
         self.precision = _abi.FP64_EXACT
+        self.nrep = 1
 
     # -- lifetime -----------------------------------------------------------
     def close(self):
@@ -106,6 +107,7 @@ class Engine:
             idp = ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         self._check(self._lib.tb3_set_particles(self._ctx, n, idp, _dp(pos), _dp(vel), _dp(acc), _dp(mass)))
         self.n = n
+        self.nrep = 1
         return self
 
     def get_particles(self):
@@ -131,6 +133,39 @@ class Engine:
         self._check(self._lib.tb3_run(self._ctx, int(steps), float(dt), _dp(gv),
                                       counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
         return counts[:steps]
+
+    # -- replica batches -------------------------------------------------------
+    def batch_set(self, offsets):
+        """Split the particles into replicas: replica r holds rows offsets[r] .. offsets[r+1]-1."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._check(self._lib.tb3_batch_set(self._ctx, len(off) - 1,
+                                            off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        self.nrep = len(off) - 1
+        return self
+
+    def batch_run(self, steps, dt, g=(0.0, 0.0, 0.0)):
+        """tb3_run with per-replica counts: an array of shape (steps, nrep)."""
+        gv = _as(g, (3,))
+        nrep = self.nrep
+        counts = np.zeros((max(int(steps), 1), nrep), dtype=np.uint64)
+        self._check(self._lib.tb3_batch_run(self._ctx, int(steps), float(dt), _dp(gv),
+                                            counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return counts[:steps]
+
+    def batch_counts(self):
+        nrep = self.nrep
+        counts = np.zeros(nrep, dtype=np.uint64)
+        self._check(self._lib.tb3_batch_counts(self._ctx, nrep, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return counts
+
+    def batch_energy(self):
+        """Per-replica energies and triplet counts of the current positions."""
+        nrep = self.nrep
+        E = np.zeros(nrep)
+        counts = np.zeros(nrep, dtype=np.uint64)
+        self._check(self._lib.tb3_batch_energy(self._ctx, nrep, _dp(E),
+                                               counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return E, counts
 
     def compute_forces(self):
         nt = ctypes.c_uint64(0)
@@ -164,6 +199,14 @@ class Engine:
                                                   ctypes.c_void_p(stream or 0),
                                                   ctypes.byref(nt) if want_count else None))
         return int(nt.value) if want_count else None
+
+    def autotune(self, reps=3):
+        """Pick the cell width / cutoff ratio and the cell kernel's resident
+        workgroups per CU by timing force sweeps of the current particles."""
+        ms, ratio, blocks = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int(0)
+        self._check(self._lib.tb3_autotune(self._ctx, int(reps), ctypes.byref(ms), ctypes.byref(ratio),
+                                           ctypes.byref(blocks)))
+        return {"sweep_ms": ms.value, "cell_ratio": ratio.value, "blocks_per_cu": blocks.value}
 
     # -- phase timing ---------------------------------------------------------
     def set_profiling(self, enable=True):
