@@ -165,11 +165,11 @@ tb3_status tb3_energy_grad_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, dou
 /* Autotuner.  No counterpart in the reference.  For the cutoff algorithm,
  * times a force sweep of the current particles (on a copy; the state is
  * untouched) for each cell width / cutoff ratio in {1, 1.15, 1.3, 1.5, 1.75,
- * 2} and each count of resident cell-kernel workgroups per CU, keeps the
- * fastest, and reports it (each output may be NULL): the best sweep time in
- * ms (HIP events, minimum of `reps` runs after one warm-up), the ratio and
- * the workgroups per CU (0 when the per-owner kernel is in charge).  The
- * all-triplet sweeps have nothing to tune and return at once. */
+ * 2}, keeps the fastest, and reports it (each output may be NULL): the best
+ * sweep time in ms (HIP events, minimum of `reps` runs after one warm-up),
+ * the ratio, and the cell kernel's resident workgroups per CU (0 when the
+ * per-owner kernel is in charge).  The all-triplet sweeps have nothing to
+ * tune and return at once. */
 tb3_status tb3_autotune(tb3_ctx *ctx, int reps, double *best_ms, double *cell_ratio, int *blocks_per_cu);
 
 /* Per-phase timing with HIP events recorded on the context's stream around

@@ -89,3 +89,39 @@ def test_gpu_exact_matches_golden():
             f = eng.get_particles()["f"]
             for row, key in enumerate(("ci", "cj", "ck")):
                 assert np.array_equal(f[row], -unhex(t[key]))
+
+
+def test_closest_packed_by_hand():
+    """ClosestPackedGenerator.cpp:15-44 on a 2 x 2 x 2 box, spacing 1, worked
+    out by hand from the loops: rows sqrt(3/4) apart, layers sqrt(2/3) apart,
+    odd layers shifted by sqrt(1/12) in y, and odd rows by 1/2 in x, where the
+    row parity runs on across layers (3 rows, then 2, then 3)."""
+    from threebodymdalgorithms_amd import generators
+    r, l, yo = np.sqrt(0.75), np.sqrt(2.0 / 3.0), np.sqrt(1.0 / 12.0)
+    expect = [(0, 0, 0), (1, 0, 0), (0.5, r, 0), (1.5, r, 0), (0, 2 * r, 0), (1, 2 * r, 0),
+              (0.5, yo, l), (1.5, yo, l), (0, yo + r, l), (1, yo + r, l),
+              (0.5, 0, 2 * l), (1.5, 0, 2 * l), (0, r, 2 * l), (1, r, 2 * l), (0.5, 2 * r, 2 * l),
+              (1.5, 2 * r, 2 * l)]
+    P = generators.closest_packed((2.0, 2.0, 2.0), 1.0)
+    assert P.shape == (16, 3)
+    assert np.allclose(P, np.array(expect, dtype=float), rtol=0, atol=1e-12)
+
+
+def test_generator_cli_writes_the_tool_format(tmp_path):
+    """particleToCSV.cpp:127-137: header, IDs from 0, ", " separators, %g."""
+    from threebodymdalgorithms_amd import generators
+    out = tmp_path / "u.csv"
+    generators.main(["--generator", "uniform", "--numparticles", "7", "--seed0", "926762934", "--blx", "10",
+                     "--bly", "10", "--blz", "10", "--mass", "1", "--vx", "0.5", "--output", str(out)])
+    lines = out.read_text().splitlines()
+    assert lines[0] == "ID, pX, pY, pZ, vX, vY, vZ, aX, aY, aZ, m"
+    P = generators.uniform(7, 10.0, 926762934)
+    for i, line in enumerate(lines[1:]):
+        cells = line.split(", ")
+        assert cells[0] == str(i)
+        assert cells[1:4] == ["%g" % v for v in P[i]]
+        assert cells[4:] == ["0.5", "0", "0", "0", "0", "0", "1"]
+    generators.main(["-generator", "grid", "-particlesperx", "2", "-particlespery", "2", "-particlesperz", "1",
+                     "-particlespacing", "1.5", "-output", str(out)])
+    rows = [line.split(", ")[1:4] for line in out.read_text().splitlines()[1:]]
+    assert rows == [["0", "0", "0"], ["1.5", "0", "0"], ["0", "1.5", "0"], ["1.5", "1.5", "0"]]

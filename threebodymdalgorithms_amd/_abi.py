@@ -8,7 +8,9 @@ import ctypes
 import os
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libtb3.so")
+# TB3_LIBRARY names another build of the same library (for example one made
+# with different compile options); the default is the in-tree build.
+LIB_PATH = os.environ.get("TB3_LIBRARY") or os.path.join(PKG_DIR, "libtb3.so")
 CLI_PATH = os.path.join(PKG_DIR, "bin", "tb3")
 
 OK, EINVAL, ENOMEM, EHIP, EUNSUPPORTED, ENODEV = range(6)

@@ -204,20 +204,29 @@ def batch256(skip_exact=False, steps=100):
     out = {"config": f"P3BCA c=2, 256 Gaussian replicas (sd 1.5), N log-uniform [256, 4096], total {len(P)}",
            "n_total": int(len(P)), "n_min": int(sizes.min()), "n_max": int(sizes.max())}
     with Engine(0) as eng:
-        for prec, k in (("fp32", steps), ("fp64", max(steps // 5, 2))):
+        for prec in ("fp32", "fp64"):
             eng.set_particles(P)
             eng.configure("p3bca", cutoff=2.0, precision=prec)
             eng.batch_set(off)
             eng.batch_run(2, 0.001, g)
+            # the force sweep of the initial state (the densest), then K steps
+            eng.set_particles(P)
+            eng.batch_set(off)
+            eng.set_profiling(True)
+            c0 = eng.compute_forces()
+            sweep_ms = eng.get_profile()["forces_ms"]
+            eng.set_profiling(False)
             eng.set_particles(P)
             eng.batch_set(off)
             t0 = time.perf_counter()
-            counts = eng.batch_run(k, 0.001, g)
+            counts = eng.batch_run(steps, 0.001, g)
             secs = time.perf_counter() - t0
             total = int(counts.sum())
-            out[prec] = {"steps": k, "ms_per_step": secs / k * 1e3, "triplets_per_step": total / k,
-                         "triplets_per_s": total / secs}
+            out[prec] = {"steps": steps, "ms_per_step": secs / steps * 1e3, "triplets_per_step": total / steps,
+                         "triplets_per_s": total / secs, "step0_sweep_ms": sweep_ms, "step0_triplets": c0,
+                         "step0_sweep_triplets_per_s": c0 / (sweep_ms * 1e-3)}
         out["fp32_speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"]
+        out["fp32_sweep_speedup_vs_fp64"] = out["fp64"]["step0_sweep_ms"] / out["fp32"]["step0_sweep_ms"]
         if not skip_exact:
             eng.set_particles(P)
             eng.configure("p3bca", cutoff=2.0, precision="exact")

@@ -122,12 +122,12 @@ struct tb3_ctx {
     long long scan_tmp_cap = 0;
     int list_cap = 128;    // LDS neighbour-list capacity per wave (power of two)
     bool cells_ok = true;  // the cell-per-workgroup kernel staged every neighbourhood so far
-    int cells_grid = 1;    // its persistent grid (CUs x resident workgroups per CU)
+    int cells_grid = 1;    // CUs x resident cell-kernel workgroups per CU (reported by the tuner)
     int dense_grid = 1;    // persistent grid of the dense-owner kernel
     int ncu = 1;           // compute units of the device
     // tuned by tb3_autotune: cell width / cutoff, resident cell-kernel workgroups per CU
     double cell_ratio = 1.0;
-    int cells_per_cu = 0;  // 0: as many as fit
+    int cells_per_cu = 0;  // resident cell-kernel workgroups per CU at the tuned width
     int *d_owners = nullptr;  // owners the cell kernel hands to the dense-owner kernel (n)
 
     // tile triples for the all-triplets fast sweep
@@ -1092,8 +1092,10 @@ tb3_status tb3_autotune(tb3_ctx *c, int reps, double *best_ms, double *cell_rati
         for (double ratio : ratios) {
             c->cell_ratio = ratio;
             configure_grid(c);
-            for (int per_cu = max_per_cu; per_cu >= 1; --per_cu) {
-                c->cells_grid = c->ncu * per_cu;
+            {
+                // the cell kernel runs one workgroup per cell; the hardware
+                // keeps max_per_cu of them resident on each CU
+                const int per_cu = max_per_cu;
                 double t = 1e300;
                 for (int r = 0; r < reps + 1; ++r) {  // the first run warms up
                     if (c->n > 0) {
@@ -1110,8 +1112,6 @@ tb3_status tb3_autotune(tb3_ctx *c, int reps, double *best_ms, double *cell_rati
                     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
                     if (r > 0) t = std::min(t, (double)ms);
                 }
-                // a neighbourhood the cell kernel cannot stage leaves the per-owner kernel in charge
-                if (!c->cells_ok) per_cu = 1;
                 if (t < best) {
                     best = t;
                     best_ratio = ratio;
@@ -1123,7 +1123,6 @@ tb3_status tb3_autotune(tb3_ctx *c, int reps, double *best_ms, double *cell_rati
         c->cell_ratio = best_ratio;
         c->cells_per_cu = best_blocks;
         configure_grid(c);
-        c->cells_grid = c->ncu * (best_blocks > 0 ? best_blocks : max_per_cu);
         if (best_ms) *best_ms = best;
         if (cell_ratio) *cell_ratio = best_ratio;
         if (blocks_per_cu) *blocks_per_cu = best_blocks;

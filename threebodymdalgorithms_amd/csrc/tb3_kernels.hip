@@ -31,6 +31,28 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 
+// v from the lane N below in the same 16-lane DPP row (row_shr:N), 0 for the
+// first N lanes of each row; done on both halves of the double.
+template <int N>
+__device__ __forceinline__ double row_shr(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x110 + N, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + N, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Inclusive sum over each 16-lane row (shift-and-add scan in the VALU, no LDS
+// crossbar); the row's total lands in its last lane (15, 31, 47, 63).
+__device__ __forceinline__ double row_sum_last(double v)
+{
+    v += row_shr<1>(v);
+    v += row_shr<2>(v);
+    v += row_shr<4>(v);
+    v += row_shr<8>(v);
+    return v;
+}
+
 // Number of set bits of m below this lane.
 __device__ __forceinline__ int lane_prefix(unsigned long long m)
 {
@@ -730,7 +752,6 @@ struct CellsDev {
     CutDev c;
     const double *xs, *ys, *zs;
     int ncell;
-    int *queue;   // [0] next cell ticket, [1] finished workgroups (both return to 0)
     int *owners;  // owners handed to the dense-owner kernel (st->dense_n of them)
 };
 
@@ -806,7 +827,16 @@ __device__ __forceinline__ int staged_slot(const int *cst, const int *cpre, int 
     return cst[k] + (t - cpre[k]);
 }
 
-template <class T, bool GRAD>
+// Cell for workgroup b: workgroups are dispatched round-robin over the 8
+// XCDs (b % 8), so each XCD gets a contiguous run of cells and the
+// neighbourhoods it stages overlap in its own L2.
+__device__ __forceinline__ int cell_for_block(int b, int ncell)
+{
+    const int chunk = (ncell + 7) >> 3;
+    return (b & 7) * chunk + (b >> 3);
+}
+
+template <class T, bool GRAD, bool WANT_E>
 __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
 {
     __shared__ double nx[NBCAP], ny[NBCAP], nz[NBCAP];
@@ -816,217 +846,190 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     __shared__ double lx[CELL_WAVES][LCAP], ly[CELL_WAVES][LCAP], lz[CELL_WAVES][LCAP];
     __shared__ double lgx[CELL_WAVES][LCAP], lgy[CELL_WAVES][LCAP], lgz[CELL_WAVES][LCAP];
     __shared__ int cst[32], cpre[33], sinfo[4];
-    __shared__ int s_cell, s_next;
+    __shared__ int s_next;
 
     const CutDev &A = a.c;
+    const int c = cell_for_block(blockIdx.x, a.ncell);
+    if (c >= a.ncell) return;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid / WAVE;
+    // Non-finite positions pair with everything: leave the sweep to the
+    // per-owner kernel.  (The binning pass set the flag, so every thread
+    // reads the same value.)
+    if (A.st->flags[0] != 0) {
+        if (tid == 0) {
+            atomicOr(&A.st->flags[2], 1);  // the update kernel skips itself
+            atomicOr(&A.st->flags[3], 1);  // the host re-runs the sweep with the per-owner kernel
+        }
+        return;
+    }
     const T nu = (T)A.nu, K = (T)(0.375 * A.nu);
     const double c2 = A.c2;
+    const SysGrid sg = grid_of_cell(A.g, c);
+    const double Lx = sg.L[0], Ly = sg.L[1], Lz = sg.L[2];
     double esum = 0.0;
     unsigned cnt = 0;
-    // Non-finite positions pair with everything: leave them to the per-owner
-    // kernel.  (The flag was set by the binning pass, so every thread reads
-    // the same value.)  The cell loop below leaves only on conditions that
-    // are uniform across the workgroup; a wave's list overflow just marks it.
-    const bool skip = A.st->flags[0] != 0;
-    bool overflow = skip;
 
-    while (!skip) {
-        if (tid == 0) s_cell = atomicAdd(&a.queue[0], 1);
+    // 1. the staged cells: c first, then its forward neighbour cells
+    if (wv == 0) stage_cells(A, sg, c, cst, cpre, sinfo);
+    __syncthreads();
+    const int total = sinfo[3], own = sinfo[1], nk = sinfo[2];
+    if (total > NBCAP) {
+        // hand every owner of c to the dense-owner kernel
+        if (tid == 0) s_next = atomicAdd(&A.st->dense_n, own);
         __syncthreads();
-        const int c = s_cell;
-        if (c >= a.ncell) break;
-        const SysGrid sg = grid_of_cell(A.g, c);
-        const double Lx = sg.L[0], Ly = sg.L[1], Lz = sg.L[2];
+        for (int t = tid; t < own; t += CELL_WAVES * WAVE) a.owners[s_next + t] = sinfo[0] + t;
+        return;
+    }
 
-        // 1. the staged cells: c first, then its forward neighbour cells
-        if (wv == 0) stage_cells(A, sg, c, cst, cpre, sinfo);
-        __syncthreads();
-        const int total = sinfo[3], own = sinfo[1], nk = sinfo[2];
-        if (total > NBCAP) {
-            // hand every owner of c to the dense-owner kernel
-            if (tid == 0) s_next = atomicAdd(&A.st->dense_n, own);
-            __syncthreads();
-            for (int t = tid; t < own; t += CELL_WAVES * WAVE) a.owners[s_next + t] = sinfo[0] + t;
-            __syncthreads();
+    // 2. stage positions and clear the accumulators
+    for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
+        const int slot = staged_slot(cst, cpre, nk, t);
+        nx[t] = a.xs[slot];
+        ny[t] = a.ys[slot];
+        nz[t] = a.zs[slot];
+        nid[t] = A.cell_idx[slot];
+        gxs[t] = 0.0;
+        gys[t] = 0.0;
+        gzs[t] = 0.0;
+    }
+    if (tid == 0) s_next = 0;
+    __syncthreads();
+
+    // 3. owners of cell c, claimed one at a time by the waves
+    unsigned short *list = wl[wv];
+    double *qx = lx[wv], *qy = ly[wv], *qz = lz[wv];
+    double *hx = lgx[wv], *hy = lgy[wv], *hz = lgz[wv];
+    for (;;) {
+        int i = 0;
+        if (lane == 0) i = atomicAdd(&s_next, 1);
+        i = __builtin_amdgcn_readfirstlane(i);
+        if (i >= own) break;
+        const double xi = nx[i], yi = ny[i], zi = nz[i];
+
+        // the owner's forward candidates within the cutoff, in staging
+        // order, copied to this wave's contiguous list
+        int len = 0;
+        for (int t0 = i + 1; t0 < total; t0 += WAVE) {
+            const int t = t0 + lane;
+            double xt = 0.0, yt = 0.0, zt = 0.0;
+            bool ok = t < total;
+            if (ok) {
+                xt = nx[t];
+                yt = ny[t];
+                zt = nz[t];
+                ok = !(sqr_dist_periodic_min(xi, yi, zi, xt, yt, zt, Lx, Ly, Lz) > c2);
+            }
+            const unsigned long long m = __ballot(ok);
+            const int pos = len + lane_prefix(m);
+            if (ok && pos < LCAP) {
+                list[pos] = (unsigned short)t;
+                qx[pos] = xt;
+                qy[pos] = yt;
+                qz[pos] = zt;
+                hx[pos] = 0.0;
+                hy[pos] = 0.0;
+                hz[pos] = 0.0;
+            }
+            len += __popcll(m);
+        }
+        if (len > LCAP) {
+            // too long for this wave's list: the dense-owner kernel takes i
+            if (lane == 0) a.owners[atomicAdd(&A.st->dense_n, 1)] = sinfo[0] + i;
             continue;
         }
+        wave_sync();
 
-        // 2. stage positions and clear the accumulators
-        for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
-            const int slot = staged_slot(cst, cpre, nk, t);
-            nx[t] = a.xs[slot];
-            ny[t] = a.ys[slot];
-            nz[t] = a.zs[slot];
-            nid[t] = A.cell_idx[slot];
-            gxs[t] = 0.0;
-            gys[t] = 0.0;
-            gzs[t] = 0.0;
-        }
-        if (tid == 0) s_next = 0;
-        __syncthreads();
-
-        // 3. owners of cell c, claimed one at a time by the waves
-        unsigned short *list = wl[wv];
-        double *qx = lx[wv], *qy = ly[wv], *qz = lz[wv];
-        double *hx = lgx[wv], *hy = lgy[wv], *hz = lgz[wv];
-        for (;;) {
-            int i = 0;
-            if (lane == 0) i = atomicAdd(&s_next, 1);
-            i = __shfl(i, 0);
-            if (i >= own) break;
-            const double xi = nx[i], yi = ny[i], zi = nz[i];
-
-            // the owner's forward candidates within the cutoff, in staging
-            // order, copied to this wave's contiguous list
-            int len = 0;
-            for (int t0 = i + 1; t0 < total; t0 += WAVE) {
-                const int t = t0 + lane;
-                double xt = 0.0, yt = 0.0, zt = 0.0;
-                bool ok = t < total;
-                if (ok) {
-                    xt = nx[t];
-                    yt = ny[t];
-                    zt = nz[t];
-                    ok = !(sqr_dist_periodic_min(xi, yi, zi, xt, yt, zt, Lx, Ly, Lz) > c2);
-                }
-                const unsigned long long m = __ballot(ok);
-                const int pos = len + lane_prefix(m);
-                if (ok && pos < LCAP) {
-                    list[pos] = (unsigned short)t;
-                    qx[pos] = xt;
-                    qy[pos] = yt;
-                    qz[pos] = zt;
-                    hx[pos] = 0.0;
-                    hy[pos] = 0.0;
-                    hz[pos] = 0.0;
-                }
-                len += __popcll(m);
-            }
-            if (len > LCAP) {
-                // too long for this wave's list: the dense-owner kernel takes i
-                if (lane == 0) a.owners[atomicAdd(&A.st->dense_n, 1)] = sinfo[0] + i;
-                continue;
-            }
-            wave_sync();
-
-            // Every unordered pair of the list once, in folded order: item
-            // t = (o - 1) len + r pairs r with (r + o) mod len, o = 1 .. H;
-            // for even len the last len / 2 items pair r with r + len / 2.
-            // Lane l takes items l, l + 64, ...; (r, o) advance by
-            // (64 mod len, 64 / len) with a carry.  Each pair is tested and,
-            // if it passes, evaluated in place.
-            double gix = 0.0, giy = 0.0, giz = 0.0;
-            const int npairs = len * (len - 1) / 2;
-            if (npairs > 0) {
-                const int q64 = WAVE / len, r64 = WAVE - q64 * len;
-                int o = lane / len;
-                int r = lane - o * len;
-                ++o;
-                for (int t0 = 0; t0 < npairs; t0 += WAVE) {
-                    if (t0 + lane < npairs) {
-                        int kk = r + o;
-                        kk = kk >= len ? kk - len : kk;
-                        const double xj = qx[r], yj = qy[r], zj = qz[r];
-                        const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
-                        if (!(sqr_dist_periodic_min(xj, yj, zj, xk, yk, zk, Lx, Ly, Lz) > c2)) {
-                            const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
-                            const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
-                            const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
-                            const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
-                            const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
-                            const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-                            T eu, ev, ew, et;
-                            at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
-                            if (GRAD) {
-                                const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
-                                gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
-                                giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
-                                giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
-                                lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
-                                lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
-                                lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
-                                lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
-                                lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
-                                lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
-                            }
-                            esum += (double)et;
-                            ++cnt;
+        // Every unordered pair of the list once, in folded order: item
+        // t = (o - 1) len + r pairs r with (r + o) mod len, o = 1 .. H;
+        // for even len the last len / 2 items pair r with r + len / 2.
+        // Lane l takes items l, l + 64, ...; (r, o) advance by
+        // (64 mod len, 64 / len) with a carry.  Each pair is tested and,
+        // if it passes, evaluated in place.
+        double gix = 0.0, giy = 0.0, giz = 0.0;
+        const int npairs = len * (len - 1) / 2;
+        if (npairs > 0) {
+            const int q64 = WAVE / len, r64 = WAVE - q64 * len;
+            int o = lane / len;
+            int r = lane - o * len;
+            ++o;
+            for (int t0 = 0; t0 < npairs; t0 += WAVE) {
+                if (t0 + lane < npairs) {
+                    int kk = r + o;
+                    kk = kk >= len ? kk - len : kk;
+                    const double xj = qx[r], yj = qy[r], zj = qz[r];
+                    const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
+                    if (!(sqr_dist_periodic_min(xj, yj, zj, xk, yk, zk, Lx, Ly, Lz) > c2)) {
+                        const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
+                        const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
+                        const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
+                        const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                        const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                        const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+                        T eu, ev, ew, et;
+                        at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
+                        if (GRAD) {
+                            const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
+                            gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
+                            giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
+                            giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
+                            lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
+                            lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
+                            lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
+                            lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
+                            lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
+                            lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
                         }
-                    }
-                    r += r64;
-                    o += q64;
-                    if (r >= len) {
-                        r -= len;
-                        ++o;
+                        if (WANT_E) esum += (double)et;
+                        ++cnt;
                     }
                 }
-            }
-            if (GRAD) {
-                // the list's gradients and the owner's go to the staged accumulators
-                wave_sync();
-                for (int t = lane; t < len; t += WAVE) {
-                    const int q = list[t];
-                    lds_add(&gxs[q], hx[t]);
-                    lds_add(&gys[q], hy[t]);
-                    lds_add(&gzs[q], hz[t]);
-                }
-                gix = wave_sum(gix);
-                giy = wave_sum(giy);
-                giz = wave_sum(giz);
-                if (lane == 0) {
-                    lds_add(&gxs[i], gix);
-                    lds_add(&gys[i], giy);
-                    lds_add(&gzs[i], giz);
+                r += r64;
+                o += q64;
+                if (r >= len) {
+                    r -= len;
+                    ++o;
                 }
             }
-            wave_sync();
         }
-        __syncthreads();
-
-        // 4. add the staged particles' gradients to global memory
         if (GRAD) {
-            for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
-                const int q = nid[t];
-                unsafeAtomicAdd(&A.fx[q], gxs[t]);
-                unsafeAtomicAdd(&A.fy[q], gys[t]);
-                unsafeAtomicAdd(&A.fz[q], gzs[t]);
+            // the owner's partial sums: 16-lane row sums in the VALU, then
+            // one LDS atomic per row into the owner's staged accumulator
+            gix = row_sum_last(gix);
+            giy = row_sum_last(giy);
+            giz = row_sum_last(giz);
+            if ((lane & 15) == 15) {
+                lds_add(&gxs[i], gix);
+                lds_add(&gys[i], giy);
+                lds_add(&gzs[i], giz);
+            }
+            // the list's gradients go to the staged accumulators
+            wave_sync();
+            for (int t = lane; t < len; t += WAVE) {
+                const int q = list[t];
+                lds_add(&gxs[q], hx[t]);
+                lds_add(&gys[q], hy[t]);
+                lds_add(&gzs[q], hz[t]);
             }
         }
-        if (A.g.nrep > 1) {
-            // replicas: this cell's sums belong to its replica
-            const unsigned long long cc = wave_sum_u64(cnt);
-            const double ee = A.want_energy ? wave_sum(esum) : 0.0;
-            if (lane == 0) {
-                if (A.want_energy) add_energy(A.st, ee);
-                add_count(A.st, A.count_slot, cc);
-                add_rep(A, sg.r, cc, ee);
-            }
-            cnt = 0;
-            esum = 0.0;
-        }
-        __syncthreads();
+        wave_sync();
     }
-
-    if (overflow && lane == 0) {
-        atomicOr(&A.st->flags[2], 1);  // the update kernel skips itself
-        atomicOr(&A.st->flags[3], 1);  // the host re-runs the sweep with the per-owner kernel
-    }
-    if (A.want_energy) esum = wave_sum(esum);
-    const unsigned long long cw = wave_sum_u64(cnt);
-    if (lane == 0) {
-        if (A.want_energy) add_energy(A.st, esum);
-        add_count(A.st, A.count_slot, cw);
-    }
-    // the last workgroup to finish returns the queue to zero for the next launch
     __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        const int done = atomicAdd(&a.queue[1], 1);
-        if (done == (int)gridDim.x - 1) {
-            atomicExch(&a.queue[0], 0);
-            atomicExch(&a.queue[1], 0);
+
+    // 4. add the staged particles' gradients to global memory
+    if (GRAD) {
+        for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
+            const int q = nid[t];
+            unsafeAtomicAdd(&A.fx[q], gxs[t]);
+            unsafeAtomicAdd(&A.fy[q], gys[t]);
+            unsafeAtomicAdd(&A.fz[q], gzs[t]);
         }
+    }
+    const unsigned long long cw = wave_sum_u64(cnt);
+    const double ee = WANT_E ? wave_sum(esum) : 0.0;
+    if (lane == 0) {
+        if (WANT_E) add_energy(A.st, ee);
+        add_count(A.st, A.count_slot, cw);
+        add_rep(A, sg.r, cw, ee);
     }
 }
 
@@ -1073,6 +1076,7 @@ __global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
     const T nu = (T)A.nu, K = (T)(0.375 * A.nu);
     const double c2 = A.c2;
     const int nowners = A.st->dense_n;  // written by the cell kernel before this launch
+    if (nowners == 0) return;           // the common case: no queue traffic at all
     double esum = 0.0;
     unsigned cnt = 0;
     bool overflow = false;
@@ -1780,6 +1784,7 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
                         int mixed, int grad, Status *st, int count_slot, int want_energy, int grid, int *owners)
 {
+    (void)grid;
     if (d.n <= 0) return;
     CellsDev a;
     a.c = cut_dev(d, g, c, nu, 0, st, count_slot);
@@ -1788,22 +1793,33 @@ void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     a.ys = g.ys;
     a.zs = g.zs;
     a.ncell = g.ncell;
-    a.queue = &st->queue[0];
     a.owners = owners;
-    const dim3 gr((unsigned)(grid < 1 ? 1 : grid)), blk(CELL_WAVES * WAVE);
+    // one workgroup per cell, rounded up to a multiple of the 8 XCDs
+    const dim3 gr((unsigned)(8 * ((g.ncell + 7) / 8))), blk(CELL_WAVES * WAVE);
+#define TB3_CELLS(T, G, E) hipLaunchKernelGGL((k_p3bca_cells<T, G, E>), gr, blk, 0, s, a)
     if (mixed) {
-        if (grad) hipLaunchKernelGGL((k_p3bca_cells<float, true>), gr, blk, 0, s, a);
-        else hipLaunchKernelGGL((k_p3bca_cells<float, false>), gr, blk, 0, s, a);
+        if (grad) {
+            if (want_energy) TB3_CELLS(float, true, true);
+            else TB3_CELLS(float, true, false);
+        } else {
+            TB3_CELLS(float, false, true);
+        }
     } else {
-        if (grad) hipLaunchKernelGGL((k_p3bca_cells<double, true>), gr, blk, 0, s, a);
-        else hipLaunchKernelGGL((k_p3bca_cells<double, false>), gr, blk, 0, s, a);
+        if (grad) {
+            if (want_energy) TB3_CELLS(double, true, true);
+            else TB3_CELLS(double, true, false);
+        } else {
+            TB3_CELLS(double, false, true);
+        }
     }
+#undef TB3_CELLS
 }
 
 int p3bca_cells_blocks_per_cu()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(k_p3bca_cells<double, true>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb,
+                                                     reinterpret_cast<const void *>(k_p3bca_cells<double, true, false>),
                                                      CELL_WAVES * WAVE, 0) != hipSuccess || nb < 1)
         nb = 1;
     return nb;
