@@ -746,7 +746,8 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
 
 constexpr int CELL_WAVES = 4;
 constexpr int NBCAP = 384;
-constexpr int LCAP = 96;
+constexpr int LCAP = 96;   // list positions fit in the 8-bit halves of a queue entry
+constexpr int RQ = 128;    // per-wave queue of passing pairs (>= 2 x 64)
 
 struct CellsDev {
     CutDev c;
@@ -845,6 +846,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     __shared__ unsigned short wl[CELL_WAVES][LCAP];
     __shared__ double lx[CELL_WAVES][LCAP], ly[CELL_WAVES][LCAP], lz[CELL_WAVES][LCAP];
     __shared__ double lgx[CELL_WAVES][LCAP], lgy[CELL_WAVES][LCAP], lgz[CELL_WAVES][LCAP];
+    __shared__ unsigned short ring[CELL_WAVES][RQ];
     __shared__ int cst[32], cpre[33], sinfo[4];
     __shared__ int s_next;
 
@@ -943,46 +945,63 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
         // t = (o - 1) len + r pairs r with (r + o) mod len, o = 1 .. H;
         // for even len the last len / 2 items pair r with r + len / 2.
         // Lane l takes items l, l + 64, ...; (r, o) advance by
-        // (64 mod len, 64 / len) with a carry.  Each pair is tested and,
-        // if it passes, evaluated in place.
-        double gix = 0.0, giy = 0.0, giz = 0.0;
+        // (64 mod len, 64 / len) with a carry.  Pairs that pass the test of
+        // their third side are queued (in item order, so the queue walks the
+        // list nearly in sequence) and evaluated 64 at a time.  Only the j
+        // and k gradients are accumulated: the owner's is minus their sum
+        // (the triplet's gradients sum to zero), taken from the list at the
+        // end.
         const int npairs = len * (len - 1) / 2;
         if (npairs > 0) {
+            unsigned short *rq = ring[wv];
+            int head = 0, qn = 0;
+            auto flush = [&](int nq) {
+                wave_sync();
+                if (lane < nq) {
+                    const unsigned e = rq[(head + lane) & (RQ - 1)];
+                    const int r = (int)(e & 0xffu), kk = (int)(e >> 8);
+                    const double xj = qx[r], yj = qy[r], zj = qz[r];
+                    const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
+                    const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
+                    const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
+                    const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
+                    const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                    const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                    const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+                    T eu, ev, ew, et;
+                    at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
+                    if (GRAD) {
+                        const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
+                        lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
+                        lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
+                        lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
+                        lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
+                        lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
+                        lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
+                    }
+                    if (WANT_E) esum += (double)et;
+                    ++cnt;
+                }
+                wave_sync();
+                head = (head + nq) & (RQ - 1);
+                qn -= nq;
+            };
             const int q64 = WAVE / len, r64 = WAVE - q64 * len;
             int o = lane / len;
             int r = lane - o * len;
             ++o;
             for (int t0 = 0; t0 < npairs; t0 += WAVE) {
+                bool ok = false;
+                int kk = 0;
                 if (t0 + lane < npairs) {
-                    int kk = r + o;
+                    kk = r + o;
                     kk = kk >= len ? kk - len : kk;
-                    const double xj = qx[r], yj = qy[r], zj = qz[r];
-                    const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
-                    if (!(sqr_dist_periodic_min(xj, yj, zj, xk, yk, zk, Lx, Ly, Lz) > c2)) {
-                        const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
-                        const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
-                        const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
-                        const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
-                        const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
-                        const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-                        T eu, ev, ew, et;
-                        at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
-                        if (GRAD) {
-                            const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
-                            gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
-                            giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
-                            giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
-                            lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
-                            lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
-                            lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
-                            lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
-                            lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
-                            lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
-                        }
-                        if (WANT_E) esum += (double)et;
-                        ++cnt;
-                    }
+                    ok = !(sqr_dist_periodic_min(qx[r], qy[r], qz[r], qx[kk], qy[kk], qz[kk], Lx, Ly, Lz) > c2);
                 }
+                const unsigned long long m = __ballot(ok);
+                if (ok) rq[(head + qn + lane_prefix(m)) & (RQ - 1)] = (unsigned short)(r | (kk << 8));
+                qn += __popcll(m);
+                if (qn >= WAVE) flush(WAVE);
                 r += r64;
                 o += q64;
                 if (r >= len) {
@@ -990,10 +1009,24 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
                     ++o;
                 }
             }
+            if (qn > 0) flush(qn);
         }
         if (GRAD) {
-            // the owner's partial sums: 16-lane row sums in the VALU, then
-            // one LDS atomic per row into the owner's staged accumulator
+            // the list's gradients go to the staged accumulators; the owner's
+            // is minus their sum (16-lane row sums in the VALU, then one LDS
+            // atomic per row)
+            wave_sync();
+            double gix = 0.0, giy = 0.0, giz = 0.0;
+            for (int t = lane; t < len; t += WAVE) {
+                const int q = list[t];
+                const double hxv = hx[t], hyv = hy[t], hzv = hz[t];
+                lds_add(&gxs[q], hxv);
+                lds_add(&gys[q], hyv);
+                lds_add(&gzs[q], hzv);
+                gix -= hxv;
+                giy -= hyv;
+                giz -= hzv;
+            }
             gix = row_sum_last(gix);
             giy = row_sum_last(giy);
             giz = row_sum_last(giz);
@@ -1001,14 +1034,6 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
                 lds_add(&gxs[i], gix);
                 lds_add(&gys[i], giy);
                 lds_add(&gzs[i], giz);
-            }
-            // the list's gradients go to the staged accumulators
-            wave_sync();
-            for (int t = lane; t < len; t += WAVE) {
-                const int q = list[t];
-                lds_add(&gxs[q], hx[t]);
-                lds_add(&gys[q], hy[t]);
-                lds_add(&gzs[q], hz[t]);
             }
         }
         wave_sync();
