@@ -840,12 +840,15 @@ __device__ __forceinline__ int cell_for_block(int b, int ncell)
 template <class T, bool GRAD, bool WANT_E>
 __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
 {
-    __shared__ double nx[NBCAP], ny[NBCAP], nz[NBCAP];
-    __shared__ double gxs[NBCAP], gys[NBCAP], gzs[NBCAP];
+    // one block per array family, so that the y, z and gradient arrays sit
+    // at constant offsets from x and their addresses fold into the LDS
+    // instructions' immediate offsets
+    __shared__ double stg[6][NBCAP];
+    double *nx = stg[0], *ny = stg[1], *nz = stg[2];
+    double *gxs = stg[3], *gys = stg[4], *gzs = stg[5];
     __shared__ int nid[NBCAP];
     __shared__ unsigned short wl[CELL_WAVES][LCAP];
-    __shared__ double lx[CELL_WAVES][LCAP], ly[CELL_WAVES][LCAP], lz[CELL_WAVES][LCAP];
-    __shared__ double lgx[CELL_WAVES][LCAP], lgy[CELL_WAVES][LCAP], lgz[CELL_WAVES][LCAP];
+    __shared__ double wls[CELL_WAVES][6][LCAP];
     __shared__ unsigned short ring[CELL_WAVES][RQ];
     __shared__ int cst[32], cpre[33], sinfo[4];
     __shared__ int s_next;
@@ -899,8 +902,8 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
 
     // 3. owners of cell c, claimed one at a time by the waves
     unsigned short *list = wl[wv];
-    double *qx = lx[wv], *qy = ly[wv], *qz = lz[wv];
-    double *hx = lgx[wv], *hy = lgy[wv], *hz = lgz[wv];
+    double *qx = wls[wv][0], *qy = wls[wv][1], *qz = wls[wv][2];
+    double *hx = wls[wv][3], *hy = wls[wv][4], *hz = wls[wv][5];
     for (;;) {
         int i = 0;
         if (lane == 0) i = atomicAdd(&s_next, 1);
