@@ -17,7 +17,8 @@ Printed JSON line:
                energy error against the bit-exact path on the initial state
   roofline     the force kernel: 115 FLOP per unique triplet / its average
                duration from HIP events on the engine's stream, against the
-               FP64 vector peak
+               FP64 vector peak; traffic = HBM bytes per launch from the
+               rocprofv3 FETCH_SIZE / WRITE_SIZE runs in profiles/
   cpu_baseline the reference-shaped CPU sweep (oracle restatement of the
                reference's one-process i<j<k loop with AoS particles and a
                virtual per-triplet call), single thread, on a bounded sample
@@ -54,9 +55,10 @@ def workload():
     return generators.csv_rounded(generators.uniform(N, BOX, SEED))
 
 
-def timed_steps(eng, P, precision, steps, warmup, sync):
+def timed_steps(eng, P, precision, steps, warmup, sync, tune=True):
     eng.set_particles(P)
     eng.configure("p3bca", cutoff=CUTOFF, precision=precision)
+    tuned = eng.autotune(reps=3) if (tune and precision != "exact") else None
     for _ in range(warmup):
         eng.step(DT, G)
     eng.set_particles(P)
@@ -85,6 +87,7 @@ def timed_steps(eng, P, precision, steps, warmup, sync):
     fk_ms = prof["forces_ms"] / max(prof["steps"], 1)
     t_avg = prof["triplets"] / max(prof["steps"], 1)
     return {
+        "tuned": tuned,
         "triplets_per_s": total / secs,
         "ms_per_step": 1e3 * secs / steps,
         "triplets_per_step": total / steps,
@@ -110,6 +113,21 @@ def fp32_accuracy(eng, P):
     return {"max_rel_force_err": float(rel.max()), "rms_rel_force_err": float(np.sqrt(np.mean(rel ** 2))),
             "max_rel_force_err_floored": float((num / np.maximum(den, floor)).max()),
             "rel_energy_err": float(abs(E32 - E64) / abs(E64))}
+
+
+PROFILE = os.path.join(ROOT, "profiles", "k_p3bca_cells_p3bca_c2_n32768_fp64.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the cutoff sweep as measured by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded in the kernel's profile summary
+    (counters cannot be read from inside this process); None if absent."""
+    try:
+        import re
+        m = re.search(r"traffic_bytes_per_launch = ([0-9.eE+]+)", open(PROFILE).read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def cpu_baseline(P, count0, target_s):
@@ -186,6 +204,7 @@ def main():
     ap.add_argument("--cpu-baseline-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-variants", action="store_true", help="time the FP64 path only")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default cell width (no tb3_autotune)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB device-to-device with torch (calibrates the HBM byte counters)")
     ap.add_argument("--only", choices=["fp64", "exact", "fp32"],
@@ -216,11 +235,11 @@ def main():
     log(f"bench: N={N} box={BOX} cutoff={CUTOFF} steps={args.steps} warmup={args.warmup}")
     if args.only:
         with tb3.Engine(0) as eng:
-            r = timed_steps(eng, P, args.only, args.steps, args.warmup, sync)
+            r = timed_steps(eng, P, args.only, args.steps, args.warmup, sync, not args.no_autotune)
         print(json.dumps({"path": args.only, **r}), flush=True)
         return
     with tb3.Engine(0) as eng:
-        fp64 = timed_steps(eng, P, "fp64", args.steps, args.warmup, sync)
+        fp64 = timed_steps(eng, P, "fp64", args.steps, args.warmup, sync, not args.no_autotune)
         log(f"bench: fp64 {fp64['ms_per_step']:.3f} ms/step {fp64['triplets_per_s']:.4e} triplets/s "
             f"force kernel {fp64['force_kernel_ms']:.3f} ms")
         out_exact = out_fp32 = None
@@ -229,7 +248,7 @@ def main():
             log(f"bench: exact {ex['ms_per_step']:.3f} ms/step force kernel {ex['force_kernel_ms']:.3f} ms")
             if ex["counts"] != fp64["counts"]:
                 log(f"bench: WARNING exact and fp64 counts differ: {ex['counts']} vs {fp64['counts']}")
-            f32 = timed_steps(eng, P, "fp32", args.steps, args.warmup, sync)
+            f32 = timed_steps(eng, P, "fp32", args.steps, args.warmup, sync, not args.no_autotune)
             acc = fp32_accuracy(eng, P)
             log(f"bench: fp32 {f32['ms_per_step']:.3f} ms/step force kernel {f32['force_kernel_ms']:.3f} ms {acc}")
             out_exact = {"value": ex["triplets_per_s"], "ms_per_step": ex["ms_per_step"],
@@ -264,9 +283,9 @@ def main():
                    "algorithm": "p3bca", "n": N, "cutoff": CUTOFF, "precision": "fp64",
                    "triplets_per_step": fp64["triplets_per_step"], "counts": fp64["counts"],
                    "force_kernel_ms": fp64["force_kernel_ms"], "neighbour_ms": fp64["neighbour_ms"],
-                   "integrate_ms": fp64["integrate_ms"]},
+                   "integrate_ms": fp64["integrate_ms"], "tuned": fp64["tuned"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": profiled_traffic(),
                      "kernel": "tb3::k_p3bca_cells<double, true, false>", "flop_per_triplet": FLOP_PER_TRIPLET},
         "cpu_baseline": cpu,
         "cpu_multicore": cpu_mc,
