@@ -53,6 +53,23 @@ __device__ __forceinline__ double row_sum_last(double v)
     return v;
 }
 
+template <int L>
+__device__ __forceinline__ double read_lane(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, L);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), L);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the wave, uniform in every lane: row sums by DPP, then the four
+// row totals read as scalars.
+__device__ __forceinline__ double wave_total(double v)
+{
+    v = row_sum_last(v);
+    return (read_lane<15>(v) + read_lane<31>(v)) + (read_lane<47>(v) + read_lane<63>(v));
+}
+
 // Number of set bits of m below this lane.
 __device__ __forceinline__ int lane_prefix(unsigned long long m)
 {
@@ -1075,7 +1092,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
 // host then re-runs the sweep with the per-owner kernel).
 
 constexpr int DENSE_WAVES = 8;
-constexpr int DCAP = 1536;
+constexpr int DCAP = 1440;  // with the 8 queues, two workgroups fit a CU's 160 KB of LDS
 
 struct DenseDev {
     CutDev c;
@@ -1097,6 +1114,7 @@ __global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
     __shared__ int cst[32], cpre[33], sinfo[4];
     __shared__ int s_item, s_len;
     __shared__ double s_gi[3];
+    __shared__ unsigned dring[DENSE_WAVES][RQ];
 
     const CutDev &A = a.c;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid / WAVE;
@@ -1164,45 +1182,64 @@ __global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
             continue;
         }
 
-        // the list's unordered pairs, folded, split over the workgroup
+        // the list's unordered pairs, folded, split over the workgroup's
+        // lanes; each wave queues its passing pairs and evaluates them 64 at
+        // a time (as the cell kernel does)
         double gix = 0.0, giy = 0.0, giz = 0.0;
         const int npairs = len * (len - 1) / 2;
         if (npairs > 0) {
+            unsigned *rq = dring[wv];
+            int head = 0, qw = 0;
+            auto flush = [&](int nq) {
+                wave_sync();
+                if (lane < nq) {
+                    const unsigned e = rq[(head + lane) & (RQ - 1)];
+                    const int r = (int)(e & 0xffffu), kk = (int)(e >> 16);
+                    const double xj = qx[r], yj = qy[r], zj = qz[r];
+                    const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
+                    const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
+                    const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
+                    const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
+                    const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                    const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                    const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+                    T eu, ev, ew, et;
+                    at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
+                    if (GRAD) {
+                        const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
+                        gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
+                        giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
+                        giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
+                        lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
+                        lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
+                        lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
+                        lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
+                        lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
+                        lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
+                    }
+                    esum += (double)et;
+                    ++cnt;
+                }
+                wave_sync();
+                head = (head + nq) & (RQ - 1);
+                qw -= nq;
+            };
             const int qn = NT / len, rn = NT - qn * len;
             int o = tid / len;
             int r = tid - o * len;
             ++o;
             for (int t0 = 0; t0 < npairs; t0 += NT) {
+                bool ok = false;
+                int kk = 0;
                 if (t0 + tid < npairs) {
-                    int kk = r + o;
+                    kk = r + o;
                     kk = kk >= len ? kk - len : kk;
-                    const double xj = qx[r], yj = qy[r], zj = qz[r];
-                    const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
-                    if (!(sqr_dist_periodic_min(xj, yj, zj, xk, yk, zk, Lx, Ly, Lz) > c2)) {
-                        const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
-                        const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
-                        const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
-                        const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
-                        const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
-                        const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-                        T eu, ev, ew, et;
-                        at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
-                        if (GRAD) {
-                            const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
-                            gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
-                            giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
-                            giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
-                            lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
-                            lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
-                            lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
-                            lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
-                            lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
-                            lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
-                        }
-                        esum += (double)et;
-                        ++cnt;
-                    }
+                    ok = !(sqr_dist_periodic_min(qx[r], qy[r], qz[r], qx[kk], qy[kk], qz[kk], Lx, Ly, Lz) > c2);
                 }
+                const unsigned long long m = __ballot(ok);
+                if (ok) rq[(head + qw + lane_prefix(m)) & (RQ - 1)] = (unsigned)r | ((unsigned)kk << 16);
+                qw += __popcll(m);
+                if (qw >= WAVE) flush(WAVE);
                 r += rn;
                 o += qn;
                 if (r >= len) {
@@ -1210,11 +1247,12 @@ __global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
                     ++o;
                 }
             }
+            if (qw > 0) flush(qw);
         }
         if (GRAD) {
-            gix = wave_sum(gix);
-            giy = wave_sum(giy);
-            giz = wave_sum(giz);
+            gix = wave_total(gix);
+            giy = wave_total(giy);
+            giz = wave_total(giz);
             if (lane == 0) {
                 lds_add(&s_gi[0], gix);
                 lds_add(&s_gi[1], giy);
@@ -1407,6 +1445,119 @@ __global__ __launch_bounds__(WAVE) void k_auta_exact(AutaExactDev a)
     if (role == 0 && lane == 0) add_count(a.st, 0, accepted);
 }
 
+// The same role streams over an owner list staged in LDS (sorted ids and
+// their positions), with the stream's candidate pairs numbered in stream
+// order and tested 64 consecutive items at a time, so every lane tests a
+// pair even when the rows are short.  Item t of a triangular stream over m
+// list entries is row a, column b > a with t = a (2m - a - 1) / 2 + (b - a - 1);
+// of the rectangular stream (role 1), t = a m2 + (b - split).  Accepted
+// pairs enter the queue in item order, which is the stream order, so the
+// fold below adds the contributions exactly as the nested loops did.
+struct StagedList {
+    const double *px, *py, *pz;
+};
+
+template <int ROLE>
+__device__ void exact_role_staged(const StagedList &L, double xp, double yp, double zp, int len, int split,
+                                  double c2, double Lx, double Ly, double Lz, double nu, ExactShared &sh, double &acc,
+                                  unsigned long long &accepted)
+{
+    const int lane = lane_id();
+    // stream geometry: rows [s, s + m), columns after the row (triangular) or [split, len)
+    int s0, m, m2;
+    if constexpr (ROLE == 0) {
+        s0 = split;
+        m = len - split;
+        m2 = 0;
+    } else if constexpr (ROLE == 1) {
+        s0 = 0;
+        m = split;
+        m2 = len - split;
+    } else {
+        s0 = 0;
+        m = split;
+        m2 = 0;
+    }
+    const long long nitems = ROLE == 1 ? (long long)m * m2 : (long long)m * (m - 1) / 2;
+    acc = 0.0;  // lanes 0, 1, 2 hold x, y, z
+    int head = 0, qn = 0;
+
+    auto flush = [&](int cnt) {
+        __syncthreads();
+        if (lane < cnt) {
+            const int q = (head + lane) & (RING - 1);
+            const int ia = sh.qa[q], ib = sh.qb[q];
+            const double xa = L.px[ia], ya = L.py[ia], za = L.pz[ia];
+            const double xb = L.px[ib], yb = L.py[ib], zb = L.pz[ib];
+            double cx, cy, cz;
+            if constexpr (ROLE == 0)
+                ref_contrib<0>(xp, yp, zp, xa, ya, za, xb, yb, zb, nu, cx, cy, cz);
+            else if constexpr (ROLE == 1)
+                ref_contrib<1>(xa, ya, za, xp, yp, zp, xb, yb, zb, nu, cx, cy, cz);
+            else
+                ref_contrib<2>(xa, ya, za, xb, yb, zb, xp, yp, zp, nu, cx, cy, cz);
+            sh.cb[lane * 3 + 0] = cx;
+            sh.cb[lane * 3 + 1] = cy;
+            sh.cb[lane * 3 + 2] = cz;
+        }
+        __syncthreads();
+        if (lane < 3) {
+            for (int t = 0; t < cnt; ++t) acc = acc - sh.cb[t * 3 + lane];
+        }
+        __syncthreads();
+        head = (head + cnt) & (RING - 1);
+        qn -= cnt;
+    };
+
+    const double twom1 = (double)(2 * m - 1);
+    const double inv_m2 = m2 > 0 ? 1.0 / (double)m2 : 0.0;
+    for (long long t0 = 0; t0 < nitems; t0 += WAVE) {
+        const long long t = t0 + lane;
+        bool ok = t < nitems;
+        int ia = 0, ib = 0;
+        if (ok) {
+            if constexpr (ROLE == 1) {
+                int ra = (int)((double)t * inv_m2);
+                long long rem = t - (long long)ra * m2;
+                while (rem < 0) {
+                    --ra;
+                    rem += m2;
+                }
+                while (rem >= m2) {
+                    ++ra;
+                    rem -= m2;
+                }
+                ia = ra;
+                ib = split + (int)rem;
+            } else {
+                // largest row a with a (2m - a - 1) / 2 <= t
+                const double disc = twom1 * twom1 - 8.0 * (double)t;
+                int ra = (int)((twom1 - __builtin_sqrt(disc > 0.0 ? disc : 0.0)) * 0.5);
+                if (ra < 0) ra = 0;
+                if (ra > m - 2) ra = m - 2;
+                auto start = [&](int r) { return (long long)r * (2 * m - r - 1) / 2; };
+                while (ra > 0 && start(ra) > t) --ra;
+                while (ra < m - 2 && start(ra + 1) <= t) ++ra;
+                ia = s0 + ra;
+                ib = ia + 1 + (int)(t - start(ra));
+            }
+            ok = !(sqr_dist_periodic_min(L.px[ia], L.py[ia], L.pz[ia], L.px[ib], L.py[ib], L.pz[ib], Lx, Ly, Lz) >
+                   c2);
+        }
+        const unsigned long long mk = __ballot(ok);
+        if (ok) {
+            const int q = (head + qn + lane_prefix(mk)) & (RING - 1);
+            sh.qa[q] = ia;
+            sh.qb[q] = ib;
+        }
+        const int got = __popcll(mk);
+        qn += got;
+        accepted += (unsigned long long)got;
+        if (qn >= WAVE) flush(WAVE);
+    }
+    if (qn > 0) flush(qn);
+}
+
 // Cutoff algorithm: one wave per owner collects N(p) from the cells, sorts it
 // by index in LDS (bitonic), then runs the three role streams in turn over
 // that one list.
@@ -1469,12 +1620,22 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
     for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o);
     const int split = below;
 
-    const LdsList list{sid_ex};
+    // stage the sorted list's positions next to it
+    double *spx = reinterpret_cast<double *>(sid_ex + cap);
+    double *spy = spx + cap, *spz = spy + cap;
+    for (int t = lane; t < len; t += WAVE) {
+        const int q = sid_ex[t];
+        spx[t] = a.x[q];
+        spy[t] = a.y[q];
+        spz[t] = a.z[q];
+    }
+    __syncthreads();
+    const StagedList list{spx, spy, spz};
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     unsigned long long accepted = 0, unused = 0;
-    exact_role<0>(a.x, a.y, a.z, p, list, len, split, true, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc0, accepted);
-    exact_role<1>(a.x, a.y, a.z, p, list, len, split, true, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc1, unused);
-    exact_role<2>(a.x, a.y, a.z, p, list, len, split, true, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc2, unused);
+    exact_role_staged<0>(list, xp, yp, zp, len, split, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc0, accepted);
+    exact_role_staged<1>(list, xp, yp, zp, len, split, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc1, unused);
+    exact_role_staged<2>(list, xp, yp, zp, len, split, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc2, unused);
     if (lane < 3) {
         a.facc[(long long)lane * a.n + p] = acc0;
         a.facc[(long long)(3 + lane) * a.n + p] = acc1;
@@ -1692,7 +1853,8 @@ size_t p3bca_fast_lds(int cap)
     return (size_t)cap * (6 * sizeof(double) + sizeof(int)) + RING * sizeof(unsigned) + 64 * sizeof(int);
 }
 
-size_t p3bca_exact_lds(int cap) { return (size_t)cap * sizeof(int); }
+// sorted ids, then (8-byte aligned, cap is a multiple of 64) the positions
+size_t p3bca_exact_lds(int cap) { return (size_t)cap * (sizeof(int) + 3 * sizeof(double)); }
 
 void launch_soa_from_rows(hipStream_t s, int n, const double *rows, double *x, double *y, double *z)
 {
