@@ -4,9 +4,10 @@ and host synchronisation between the replays:
   * exact path: every state array and every count bit-identical;
   * FP64 path: counts identical, state within 1e-12 (its force sums are
     atomic, so their order is free);
-  * a run in which a sweep overflows part way (two clusters that merge):
+  * runs in which a sweep overflows part way (two clusters that merge):
     the halted replays leave the state alone and the host finishes the step,
-    so the result still equals the one-step-at-a-time run bit for bit.
+    so the result still equals the one-step-at-a-time run (bit for bit on the
+    exact path), also where the dense-owner kernel is the one that overflows.
 """
 import sys
 
@@ -130,6 +131,42 @@ def test_run_with_overflow_midway(precision):
         got, cgot = state_after(e2, P, V, "p3bca", 2.0, precision, [("run", 9)], L=L, nu=1e-9)
     assert max(cref) > 3 * cref[0]  # the neighbourhoods did grow
     assert_same(got, cgot, ref, cref, precision == "exact")
+
+
+def lattice_cluster(n, h, centre, rng):
+    """n sites of a cubic lattice of spacing h nearest the centre, jittered by
+    at most h/10: a dense cluster without close pairs."""
+    k = int(np.ceil((n * 6 / np.pi) ** (1 / 3))) + 2
+    g = (np.arange(k) - (k - 1) / 2) * h
+    X = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    X = X[np.argsort((X ** 2).sum(1), kind="stable")[:n]]
+    return X + rng.uniform(-0.1 * h, 0.1 * h, X.shape) + centre
+
+
+def test_run_with_dense_overflow_midway():
+    """Two clusters of 800 (radius ~0.56, cutoff 2): every owner has 799
+    neighbours, more than the cell kernel's lists hold, so all owners go to
+    the dense-owner kernel.  At step 5 the clusters overlap and an owner has
+    ~1596 neighbours, more than the dense-owner kernel stages: the sweep
+    overflows, the replays after it halt (their sweep kernels must do nothing,
+    and the dense-owner queue must not grow across them) and the host
+    finishes with the per-owner kernel.  nu = 1e-12 keeps the forces below 1,
+    so the drift alone brings the clusters together."""
+    rng = np.random.default_rng(12)
+    P = np.concatenate([lattice_cluster(800, 0.1, np.array([6.0, 10.0, 10.0]), rng),
+                        lattice_cluster(800, 0.1, np.array([14.0, 10.0, 10.0]), rng)])
+    V = np.zeros_like(P)
+    V[:800, 0] = 600.0
+    V[800:, 0] = -600.0
+    L = np.array([20.0, 20.0, 20.0])
+    say("run with a dense-owner overflow midway fp64")
+    with tb3.Engine(0) as e1:
+        ref, cref = state_after(e1, P, V, "p3bca", 2.0, "fp64", [("step", 1)] * 9, L=L, nu=1e-12)
+    with tb3.Engine(0) as e2:
+        got, cgot = state_after(e2, P, V, "p3bca", 2.0, "fp64", [("run", 9)], L=L, nu=1e-12)
+    assert cref[0] == 2 * (800 * 799 * 798 // 6)  # each cluster: all its triplets
+    assert max(cref) == 1600 * 1599 * 1598 // 6   # merged: all triplets
+    assert_same(got, cgot, ref, cref, False)
 
 
 def test_autotune_keeps_results(eng):

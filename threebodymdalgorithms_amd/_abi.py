@@ -77,6 +77,11 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise TB3Error(ENODEV, f"{LIB_PATH} is not built; run threebodymdalgorithms_amd.build.build()")
+    # torch carries its own HIP runtime.  Loaded first, it provides the
+    # libamdhip64.so.7 the library needs, and the process has one runtime;
+    # loaded after the library, it would bring a second HSA runtime into the
+    # process, which then finds no GPU for torch.
+    import torch  # noqa: F401
     try:
         lib = ctypes.CDLL(LIB_PATH)
     except OSError as e:

@@ -131,10 +131,10 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
                        int cap, int mixed, int grad, Status *st, int count_slot, int want_energy);
 void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
                         int cap, Status *st);
-// Fast cutoff sweep with one cell per workgroup (persistent grid of `grid`
-// workgroups over a device queue of cells).  Sets st->flags[2] and [3] when a
-// neighbourhood does not fit its LDS staging; the caller then re-runs the
-// sweep with launch_p3bca_fast.
+// Fast cutoff sweep with one workgroup per cell (`grid` is unused).  Owners
+// it cannot stage go to `owners` for launch_p3bca_dense, which must follow on
+// the same stream.  Non-finite positions set st->flags[2] and [3]; the caller
+// then re-runs the sweep with launch_p3bca_fast.
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
                         int mixed, int grad, Status *st, int count_slot, int want_energy, int grid, int *owners);
 // Workgroups of launch_p3bca_cells that fit on one CU at once.

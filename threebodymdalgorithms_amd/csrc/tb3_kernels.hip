@@ -136,6 +136,10 @@ __global__ void k_rows_from_soa(int n, const double *__restrict__ x, const doubl
 __global__ void k_reset_status(Status *st, int keep_nonfinite, int in_graph)
 {
     if (threadIdx.x == 0) {
+        // the dense-owner queue length is per sweep; a halted replay (whose
+        // cell and dense-owner kernels return at once) keeps every other
+        // field for the host
+        st->dense_n = 0;
         if (in_graph && st->halt) return;
         st->halt = 0;
         for (int r = 0; r < st->nrep && st->nrep > 1; ++r) {
@@ -874,9 +878,11 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     const int c = cell_for_block(blockIdx.x, a.ncell);
     if (c >= a.ncell) return;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid / WAVE;
-    // Non-finite positions pair with everything: leave the sweep to the
-    // per-owner kernel.  (The binning pass set the flag, so every thread
-    // reads the same value.)
+    // A halted step-graph replay leaves the step to the host.  Non-finite
+    // positions pair with everything: leave the sweep to the per-owner
+    // kernel.  (Earlier launches set both, so every thread reads the same
+    // values.)
+    if (A.st->halt) return;
     if (A.st->flags[0] != 0) {
         if (tid == 0) {
             atomicOr(&A.st->flags[2], 1);  // the update kernel skips itself
@@ -1121,8 +1127,8 @@ __global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
     constexpr int NT = DENSE_WAVES * WAVE;
     const T nu = (T)A.nu, K = (T)(0.375 * A.nu);
     const double c2 = A.c2;
-    const int nowners = A.st->dense_n;  // written by the cell kernel before this launch
-    if (nowners == 0) return;           // the common case: no queue traffic at all
+    const int nowners = A.st->halt ? 0 : A.st->dense_n;  // written by the cell kernel before this launch
+    if (nowners == 0) return;                            // the common case: no queue traffic at all
     double esum = 0.0;
     unsigned cnt = 0;
     bool overflow = false;
