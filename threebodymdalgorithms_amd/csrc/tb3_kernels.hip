@@ -1066,13 +1066,19 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     }
     __syncthreads();
 
-    // 4. add the staged particles' gradients to global memory
+    // 4. add the staged particles' gradients to global memory.  Staged
+    // particles that no owner of c paired with (many in the edge and corner
+    // cells) hold exact zeros and are skipped: f starts at +0.0, and adding
+    // +-0.0 to it never changes it.
     if (GRAD) {
         for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
             const int q = nid[t];
-            unsafeAtomicAdd(&A.fx[q], gxs[t]);
-            unsafeAtomicAdd(&A.fy[q], gys[t]);
-            unsafeAtomicAdd(&A.fz[q], gzs[t]);
+            const double gx = gxs[t], gy = gys[t], gz = gzs[t];
+            if (gx != 0.0 || gy != 0.0 || gz != 0.0) {
+                unsafeAtomicAdd(&A.fx[q], gx);
+                unsafeAtomicAdd(&A.fy[q], gy);
+                unsafeAtomicAdd(&A.fz[q], gz);
+            }
         }
     }
     const unsigned long long cw = wave_sum_u64(cnt);
