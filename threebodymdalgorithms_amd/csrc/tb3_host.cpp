@@ -380,7 +380,9 @@ int next_pow2(int v)
 // algorithm, on a cell list already built for them).
 //   grad   : dE/dx (the reference's f) into d.fx/fy/fz
 //   energy : E into the status block
-void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool energy)
+// The sweep of a pass.  With `tail` it ends with the role sum (exact path)
+// and the status fold; an integrating pass leaves both to launch_tail.
+void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool energy, bool tail)
 {
     const bool cut = uses_cutoff(c);
     const bool exact = c->precision == TB3_FP64_EXACT;
@@ -410,21 +412,35 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
         }
     }
     if (c->recording) HIPCHK(hipEventRecord(c->ev[3], s));
-    if (exact && grad && d.n >= 3) launch_sum_roles(s, d);
-    launch_finalize(s, c->d_status);
+    if (tail) {
+        if (exact && grad && d.n >= 3) launch_sum_roles(s, d);
+        launch_finalize(s, c->d_status);
+    }
     HIPCHK(hipGetLastError());
 }
 
-// Prepares a sweep over d: resets the status block, runs the predictor when
-// asked (fused into the binning pass for the cutoff algorithm), zeroes the
-// force where the sweep accumulates atomically and builds the cell list.
+// The end of an integrating pass after launch_sweep(..., tail = false): one
+// kernel for the role sum, the status fold and the update (and, in a step
+// graph, the count record and the status clear for the next replay).
+void launch_tail(tb3_ctx *c, hipStream_t s, DeviceState &d, double dt, const double *g, bool grad, bool record)
+{
+    const bool sum_roles = c->precision == TB3_FP64_EXACT && grad && d.n >= 3;
+    launch_update_fin(s, d, dt * 0.5, g, c->d_status, sum_roles ? 1 : 0, record ? 1 : 0);
+    HIPCHK(hipGetLastError());
+}
+
+// Prepares a sweep over d: resets the status block (not inside a step
+// graph, whose previous replay, or k_begin_steps, cleared it), runs the
+// predictor when asked (fused into the binning pass for the cutoff
+// algorithm), zeroes the force where the sweep accumulates atomically and
+// builds the cell list.
 void launch_prepare(tb3_ctx *c, hipStream_t s, DeviceState &d, bool predict, double dt, bool grad,
                     bool in_graph = false)
 {
     const bool cut = uses_cutoff(c);
     const bool exact = c->precision == TB3_FP64_EXACT;
     const bool zero_f = grad && (!exact || d.n < 3);
-    launch_reset_status(s, c->d_status, 0, in_graph ? 1 : 0);
+    if (!in_graph) launch_reset_status(s, c->d_status, 0);
     if (c->recording) HIPCHK(hipEventRecord(c->ev[0], s));
     if (cut && d.n >= 3) {
         // the predictor is fused into the binning pass: its time is counted
@@ -488,8 +504,8 @@ void finish_overflowed(tb3_ctx *c, hipStream_t s, DeviceState &d, bool integrate
             launch_zero(s, d.fy, d.n);
             launch_zero(s, d.fz, d.n);
         }
-        launch_sweep(c, s, d, grad, energy);
-        if (integrate) launch_update(s, d, dt * 0.5, g, c->d_status);
+        launch_sweep(c, s, d, grad, energy, !integrate);
+        if (integrate) launch_tail(c, s, d, dt, g, grad, false);
         read_status(c, s);
     }
 }
@@ -505,8 +521,8 @@ unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool inte
         ~Reset() { c->recording = false; }
     } reset{c};
     launch_prepare(c, s, d, integrate, dt, grad);
-    launch_sweep(c, s, d, grad, energy);
-    if (integrate) launch_update(s, d, dt * 0.5, g, c->d_status);
+    launch_sweep(c, s, d, grad, energy, !integrate);
+    if (integrate) launch_tail(c, s, d, dt, g, grad, false);
     if (c->recording) HIPCHK(hipEventRecord(c->ev[4], s));
     read_status(c, s);
     finish_overflowed(c, s, d, integrate, dt, g, grad, energy);
@@ -543,9 +559,8 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
         launch_prepare(c, s, d, true, dt, true, true);
-        launch_sweep(c, s, d, true, false);
-        launch_update(s, d, dt * 0.5, g, c->d_status);
-        launch_record_count(s, c->d_status);
+        launch_sweep(c, s, d, true, false, false);
+        launch_tail(c, s, d, dt, g, true, true);
     } catch (...) {
         (void)hipStreamEndCapture(s, &graph);
         if (graph) (void)hipGraphDestroy(graph);

@@ -92,19 +92,21 @@ struct Status {
 void launch_soa_from_rows(hipStream_t s, int n, const double *rows, double *x, double *y, double *z);
 void launch_rows_from_soa(hipStream_t s, int n, const double *x, const double *y, const double *z, double *rows);
 // Clears the status block (and halt); keep_nonfinite preserves flags[0].
-// in_graph: the reset belongs to a replayed step graph and does nothing
-// while st->halt is set.
-void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite, int in_graph = 0);
-// Starts a run of replayed step graphs: halt = 0, nstep = 0, counts = counts.
+void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite);
+// Starts a run of replayed step graphs: clears the status block, halt = 0,
+// nstep = 0, counts = counts.
 void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts);
-// Appends st->count[0] to st->counts unless the step halted.
-void launch_record_count(hipStream_t s, Status *st);
 // Folds the slot partial sums into st->count / st->energy and clears the
 // slots; sets st->halt when the sweep overflowed (st->flags[2]).
 void launch_finalize(hipStream_t s, Status *st);
 void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h);
-// a = g - f/m, v += a dt/2; skipped entirely when st->flags[2] (overflow) is set.
-void launch_update(hipStream_t s, const DeviceState &d, double hdt, const double g[3], const Status *st);
+// The last kernel of an integrating pass: launch_finalize's fold, then
+// a = g - f/m, v += a dt/2 (f = b0 + (b1 + b2) first with sum_roles), the
+// update skipped entirely when st->flags[2] (overflow) is set.  With record
+// (inside a replayed step graph) a step that did not halt also appends its
+// count to st->counts and clears the status block for the next replay.
+void launch_update_fin(hipStream_t s, const DeviceState &d, double hdt, const double g[3], Status *st, int sum_roles,
+                       int record);
 void launch_sum_roles(hipStream_t s, const DeviceState &d);
 void launch_zero(hipStream_t s, double *p, long long count);
 

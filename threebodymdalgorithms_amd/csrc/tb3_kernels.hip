@@ -133,51 +133,44 @@ __global__ void k_rows_from_soa(int n, const double *__restrict__ x, const doubl
     rows[3 * (long long)p + 2] = z[p];
 }
 
-__global__ void k_reset_status(Status *st, int keep_nonfinite, int in_graph)
+// The per-pass fields of the status block (one thread).
+__device__ void clear_status(Status *st, bool keep_nonfinite)
+{
+    for (int r = 0; r < st->nrep && st->nrep > 1; ++r) {
+        st->rep_count[r] = 0;
+        st->rep_energy[r] = 0.0;
+    }
+    st->count[0] = 0;
+    st->count[1] = 0;
+    st->energy = 0.0;
+    st->dense_n = 0;
+    if (!keep_nonfinite) st->flags[0] = 0;
+    st->flags[1] = st->flags[2] = st->flags[3] = 0;
+}
+
+__global__ void k_reset_status(Status *st, int keep_nonfinite)
 {
     if (threadIdx.x == 0) {
-        // the dense-owner queue length is per sweep; a halted replay (whose
-        // cell and dense-owner kernels return at once) keeps every other
-        // field for the host
-        st->dense_n = 0;
-        if (in_graph && st->halt) return;
         st->halt = 0;
-        for (int r = 0; r < st->nrep && st->nrep > 1; ++r) {
-            st->rep_count[r] = 0;
-            st->rep_energy[r] = 0.0;
-        }
-        st->count[0] = 0;
-        st->count[1] = 0;
-        st->energy = 0.0;
-        st->dense_n = 0;
-        if (!keep_nonfinite) st->flags[0] = 0;
-        st->flags[1] = st->flags[2] = st->flags[3] = 0;
+        clear_status(st, keep_nonfinite != 0);
     }
 }
 
+// A run of replayed step graphs starts from a clear status block; each
+// replay's last kernel clears it again for the next (see k_update_fin).
 __global__ void k_begin_steps(Status *st, unsigned long long *counts)
 {
     if (threadIdx.x == 0) {
+        clear_status(st, false);
         st->halt = 0;
         st->nstep = 0;
         st->counts = counts;
     }
 }
 
-__global__ void k_record_count(Status *st)
-{
-    if (threadIdx.x == 0 && !st->halt) {
-        if (st->nrep > 1) {
-            for (int r = 0; r < st->nrep; ++r) st->counts[(long long)st->nstep * st->nrep + r] = st->rep_count[r];
-        } else {
-            st->counts[st->nstep] = st->count[0];
-        }
-        st->nstep = st->nstep + 1;
-    }
-}
-
-// Folds the slot partial sums into the status block and clears the slots.
-__global__ __launch_bounds__(NSLOT) void k_finalize(Status *st)
+// Folds the slot partial sums into the status block and clears the slots
+// (all NSLOT threads of one workgroup); sets halt when the sweep overflowed.
+__device__ void fold_slots(Status *st)
 {
     __shared__ unsigned long long c0[NSLOT / WAVE], c1[NSLOT / WAVE];
     __shared__ double e0[NSLOT / WAVE];
@@ -207,6 +200,8 @@ __global__ __launch_bounds__(NSLOT) void k_finalize(Status *st)
     }
 }
 
+__global__ __launch_bounds__(NSLOT) void k_finalize(Status *st) { fold_slots(st); }
+
 // Particle::UpdatePredictorStage (utility/structs.hpp:40-55):
 // p = (p + v*dt) + aX*h with h = (dt*dt)*0.5, aX for all three axes.
 __device__ __forceinline__ void predict_one(const DeviceState &d, int p, double dt, double h)
@@ -224,14 +219,50 @@ __global__ void k_predict(DeviceState d, double dt, double h)
     predict_one(d, p, dt, h);
 }
 
-// Particle::Update (structs.hpp:57-99): a = g - f/m, v = v + a*(dt*0.5).
-__global__ void k_update(DeviceState d, double hdt, double gx, double gy, double gz, const Status *st)
+// The last kernel of an integrating pass.  Workgroup 0 first folds the
+// sweep's slot sums into the status block (k_finalize); with `record` (a
+// replayed step graph) it then appends the step's count and, unless the
+// step halted, clears the status block for the next replay.  Every thread
+// then applies Particle::Update (structs.hpp:57-99) to its particle:
+// a = g - f/m, v = v + a*(dt*0.5), with f = b0 + (b1 + b2) first when
+// `sum_roles` (Algorithm::SumUpParticles, Algorithm.cpp:245-249).  Nothing
+// is updated when the sweep overflowed (the host re-runs it first); the
+// status clear cannot affect that test, since it happens only when
+// flags[2] is already zero.
+__global__ __launch_bounds__(NSLOT) void k_update_fin(DeviceState d, double hdt, double gx, double gy, double gz,
+                                                     Status *st, int sum_roles, int record)
 {
-    if (st && st->flags[2]) return;  // the sweep overflowed: the host re-runs it first
+    if (blockIdx.x == 0) {
+        fold_slots(st);
+        if (record && threadIdx.x == 0 && !st->halt) {
+            if (st->nrep > 1) {
+                for (int r = 0; r < st->nrep; ++r) st->counts[(long long)st->nstep * st->nrep + r] = st->rep_count[r];
+            } else {
+                st->counts[st->nstep] = st->count[0];
+            }
+            st->nstep = st->nstep + 1;
+            clear_status(st, false);
+        }
+    }
+    if (st->flags[2]) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.n) return;
+    double fx, fy, fz;
+    if (sum_roles) {
+        const long long n = d.n;
+        fx = d.facc[0 * n + p] + (d.facc[3 * n + p] + d.facc[6 * n + p]);
+        fy = d.facc[1 * n + p] + (d.facc[4 * n + p] + d.facc[7 * n + p]);
+        fz = d.facc[2 * n + p] + (d.facc[5 * n + p] + d.facc[8 * n + p]);
+        d.fx[p] = fx;
+        d.fy[p] = fy;
+        d.fz[p] = fz;
+    } else {
+        fx = d.fx[p];
+        fy = d.fy[p];
+        fz = d.fz[p];
+    }
     const double m = d.m[p];
-    const double a0 = gx - d.fx[p] / m, a1 = gy - d.fy[p] / m, a2 = gz - d.fz[p] / m;
+    const double a0 = gx - fx / m, a1 = gy - fy / m, a2 = gz - fz / m;
     d.vx[p] = d.vx[p] + a0 * hdt;
     d.vy[p] = d.vy[p] + a1 * hdt;
     d.vz[p] = d.vz[p] + a2 * hdt;
@@ -1880,9 +1911,9 @@ void launch_rows_from_soa(hipStream_t s, int n, const double *x, const double *y
     hipLaunchKernelGGL(k_rows_from_soa, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, y, z, rows);
 }
 
-void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite, int in_graph)
+void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite)
 {
-    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, s, st, keep_nonfinite, in_graph);
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, s, st, keep_nonfinite);
 }
 
 void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts)
@@ -1890,7 +1921,6 @@ void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts)
     hipLaunchKernelGGL(k_begin_steps, dim3(1), dim3(64), 0, s, st, counts);
 }
 
-void launch_record_count(hipStream_t s, Status *st) { hipLaunchKernelGGL(k_record_count, dim3(1), dim3(64), 0, s, st); }
 
 void launch_finalize(hipStream_t s, Status *st) { hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NSLOT), 0, s, st); }
 
@@ -1900,10 +1930,11 @@ void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h)
     hipLaunchKernelGGL(k_predict, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, dt, h);
 }
 
-void launch_update(hipStream_t s, const DeviceState &d, double hdt, const double g[3], const Status *st)
+void launch_update_fin(hipStream_t s, const DeviceState &d, double hdt, const double g[3], Status *st, int sum_roles,
+                       int record)
 {
-    if (d.n <= 0) return;
-    hipLaunchKernelGGL(k_update, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, hdt, g[0], g[1], g[2], st);
+    const unsigned nb = d.n > 0 ? blocks_for(d.n, NSLOT) : 1;
+    hipLaunchKernelGGL(k_update_fin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g[0], g[1], g[2], st, sum_roles, record);
 }
 
 void launch_sum_roles(hipStream_t s, const DeviceState &d)
