@@ -122,6 +122,7 @@ struct tb3_ctx {
     long long scan_tmp_cap = 0;
     int list_cap = 128;    // LDS neighbour-list capacity per wave (power of two)
     bool cells_ok = true;  // the cell-per-workgroup kernel staged every neighbourhood so far
+    bool dense_on = false;  // the dense-owner kernel follows the cell kernel (once some owner needed it)
     int cells_grid = 1;    // CUs x resident cell-kernel workgroups per CU (reported by the tuner)
     int dense_grid = 1;    // persistent grid of the dense-owner kernel
     int ncu = 1;           // compute units of the device
@@ -146,6 +147,7 @@ struct tb3_ctx {
     double graph_dt = 0.0, graph_g[3] = {0, 0, 0};
     int graph_cap = -1;
     bool graph_cells_ok = false;
+    bool graph_dense_on = false;
     unsigned long long *d_counts = nullptr;
     long long counts_cap = 0;
 
@@ -297,6 +299,7 @@ void configure_grid(tb3_ctx *c)
     }
     publish_replicas(c);
     c->cells_ok = true;  // a new grid gets a new chance at the cell kernel
+    c->dense_on = false;
     ++c->epoch;
 }
 
@@ -400,9 +403,11 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
             const int slot = (exact && grad) ? 1 : 0;  // the exact pass owns count[0]
             if (cut && c->cells_ok) {
                 launch_p3bca_cells(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
-                                   energy ? 1 : 0, std::min(c->cells_grid, c->grid.ncell), c->d_owners);
-                launch_p3bca_dense(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
-                                   energy ? 1 : 0, c->d_owners, c->dense_grid);
+                                   energy ? 1 : 0, std::min(c->cells_grid, c->grid.ncell), c->d_owners,
+                                   c->dense_on ? 1 : 0);
+                if (c->dense_on)
+                    launch_p3bca_dense(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
+                                       energy ? 1 : 0, c->d_owners, c->dense_grid);
             }
             else if (cut)
                 launch_p3bca_fast(s, d, c->grid, cs, c->nu, c->list_cap, mixed, fast_grad ? 1 : 0, c->d_status,
@@ -469,6 +474,8 @@ bool grow_after_overflow(tb3_ctx *c)
     // a neighbourhood too large for the cell kernel's staging: use the
     // per-owner kernel for this particle system from now on
     if (c->h_status->flags[3]) c->cells_ok = false;
+    // owners that need the dense-owner kernel: launch it from now on
+    if (c->h_status->want_dense) c->dense_on = true;
     const int need = c->h_status->flags[1];
     if (need > c->list_cap) {
         if (need > MAX_LIST)
@@ -547,7 +554,7 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
 {
     if (c->step_exec && c->graph_epoch == c->epoch && c->graph_dt == dt && c->graph_g[0] == g[0] &&
         c->graph_g[1] == g[1] && c->graph_g[2] == g[2] && c->graph_cap == c->list_cap &&
-        c->graph_cells_ok == c->cells_ok)
+        c->graph_cells_ok == c->cells_ok && c->graph_dense_on == c->dense_on)
         return;
     if (c->step_exec) {
         (void)hipGraphExecDestroy(c->step_exec);
@@ -578,6 +585,7 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
     for (int k = 0; k < 3; ++k) c->graph_g[k] = g[k];
     c->graph_cap = c->list_cap;
     c->graph_cells_ok = c->cells_ok;
+    c->graph_dense_on = c->dense_on;
 }
 
 // The last pass's accepted-triplet count, or per-replica counts for a batch.

@@ -144,6 +144,7 @@ __device__ void clear_status(Status *st, bool keep_nonfinite)
     st->count[1] = 0;
     st->energy = 0.0;
     st->dense_n = 0;
+    st->want_dense = 0;
     if (!keep_nonfinite) st->flags[0] = 0;
     st->flags[1] = st->flags[2] = st->flags[3] = 0;
 }
@@ -805,8 +806,18 @@ struct CellsDev {
     CutDev c;
     const double *xs, *ys, *zs;
     int ncell;
-    int *owners;  // owners handed to the dense-owner kernel (st->dense_n of them)
+    int *owners;   // owners handed to the dense-owner kernel (st->dense_n of them)
+    int dense_on;  // 0: the dense-owner kernel does not follow; an owner that needs it fails the pass
 };
+
+// An owner that needs the dense-owner kernel when none follows this launch:
+// the pass fails (the update skips itself) and asks the host for a re-run
+// with the dense-owner kernel.
+__device__ __forceinline__ void want_dense_kernel(Status *st)
+{
+    atomicOr(&st->want_dense, 1);
+    atomicOr(&st->flags[2], 1);
+}
 
 // The staged cells of cell c (one wave): c first, then its forward
 // neighbour cells.  Publishes each cell's (start, staged prefix) in cst/cpre,
@@ -934,6 +945,10 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     const int total = sinfo[3], own = sinfo[1], nk = sinfo[2];
     if (total > NBCAP) {
         // hand every owner of c to the dense-owner kernel
+        if (!a.dense_on) {
+            if (tid == 0) want_dense_kernel(A.st);
+            return;
+        }
         if (tid == 0) s_next = atomicAdd(&A.st->dense_n, own);
         __syncthreads();
         for (int t = tid; t < own; t += CELL_WAVES * WAVE) a.owners[s_next + t] = sinfo[0] + t;
@@ -993,7 +1008,10 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
         }
         if (len > LCAP) {
             // too long for this wave's list: the dense-owner kernel takes i
-            if (lane == 0) a.owners[atomicAdd(&A.st->dense_n, 1)] = sinfo[0] + i;
+            if (lane == 0) {
+                if (a.dense_on) a.owners[atomicAdd(&A.st->dense_n, 1)] = sinfo[0] + i;
+                else want_dense_kernel(A.st);
+            }
             continue;
         }
         wave_sync();
@@ -2015,11 +2033,13 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
 }
 
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid, int *owners)
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid, int *owners,
+                        int dense_on)
 {
     (void)grid;
     if (d.n <= 0) return;
     CellsDev a;
+    a.dense_on = dense_on;
     a.c = cut_dev(d, g, c, nu, 0, st, count_slot);
     a.c.want_energy = want_energy;
     a.xs = g.xs;
