@@ -126,7 +126,10 @@ def test_exact_trajectory_bit_identical(eng, oracle):
         assert np.array_equal(a, b), name
 
 
-@pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0)])
+# 12.6: 6 cells per dimension (the cutoff tests keep the periodic form);
+# 16.0: 7 cells, where the plain-difference form serves the inner cells
+@pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0),
+                                               ("p3bca", 8192, 16.0, 2.0)])
 def test_fast_fp64_one_sweep(eng, oracle, algo, n, box, cutoff):
     P = uniform_particles(n, box, 300 + n)
     L, _ = oracle.domain_size(P)
@@ -140,7 +143,38 @@ def test_fast_fp64_one_sweep(eng, oracle, algo, n, box, cutoff):
     assert e["max_rel"] <= 1e-10
 
 
-@pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fast_sweep_particles_outside_box(eng, oracle, precision):
+    """A fixed 16^3 domain (8 cells per dimension) with a fifth of the
+    particles moved out of the box by +-L in one dimension: they stay in
+    their wrapped cells, the cutoff test is periodic and the force uses the
+    raw positions (Algorithm.cpp, AxilrodTeller.cpp).  Workgroups that
+    stage such a particle must keep the periodic cutoff form; the counts
+    must still match the oracle exactly."""
+    rng = np.random.default_rng(77)
+    L = np.array([16.0, 16.0, 16.0])
+    P = uniform_particles(8192, 16.0, 7001)
+    moved = rng.random(len(P)) < 0.2
+    dim = rng.integers(0, 3, len(P))
+    sign = rng.choice([-1.0, 1.0], len(P))
+    P[moved, dim[moved]] += sign[moved] * 16.0
+    say(f"{precision} p3bca n=8192 with {int(moved.sum())} particles outside the box")
+    f, cnt = gpu_forces(eng, P, "p3bca", 2.0, precision, L=L)
+    *_, fo, co = oracle.sweep(P, cutoff=2.0, L=L, mode="owner")
+    assert cnt == co
+    e = force_errors(f, fo)
+    say(f"  errors {e}")
+    # the moved particles' forces come from raw vectors about a box length
+    # long and largely cancel, which inflates the pure ratio |df|/|F| of a
+    # few particles; the ratio floored at the RMS force carries the bar
+    if precision == "fp64":
+        assert e["max_rel_floored"] <= 1e-12 and e["rms_rel"] <= 1e-10
+    else:
+        assert e["max_rel_floored"] <= 1e-5 and e["rms_rel"] <= 1e-3
+
+
+@pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0),
+                                               ("p3bca", 8192, 16.0, 2.0)])
 def test_fp32_mixed_one_sweep(eng, oracle, algo, n, box, cutoff):
     P = uniform_particles(n, box, 500 + n)
     L, _ = oracle.domain_size(P)

@@ -4,6 +4,7 @@
 // expression rounding as written; the fast paths spell their fused
 // multiply-adds out with fma().
 #include <climits>
+#include <type_traits>
 
 #include "at_math.h"
 #include "tb3_internal.h"
@@ -783,22 +784,25 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
 // Ownership: each accepted triplet belongs to the one particle that sees both
 // others "forward": in its own cell at a later slot, or in a forward
 // neighbour cell (the half shell of positive stencil offsets, or of larger
-// cell index on grids with a 3-cell dimension; see below).  A workgroup takes
-// cell c from a device queue and stages in LDS c's particles followed by
-// those of c's forward cells: every particle forward of an owner in c and
-// within the cutoff of it is in this range.  Each
-// wave then claims owners i of cell c, collects i's candidates t > i within
-// the cutoff (a contiguous LDS range, no search) into a contiguous per-wave
-// list, then walks the list's unordered pairs in folded order, 64 at a time,
-// testing the third side and evaluating each passing triplet in place.  The
-// j and k gradients go to per-wave list accumulators (LDS atomics), folded
-// into the staged particles' accumulators once per owner; the workgroup adds
-// those to global memory once per staged particle.  The owners of a
-// neighbourhood larger than NBCAP, and an owner whose list is longer than
-// LCAP, are queued for the dense-owner kernel below instead.
+// cell index on grids with a 3-cell dimension; see below).  One workgroup
+// per cell c stages in LDS c's particles followed by those particles of c's
+// forward cells that lie within the cutoff of c's box: every particle
+// forward of an owner in c and within the cutoff of it is in this range.
+// Each wave then claims owners i of cell c, collects i's candidates t > i
+// within the cutoff (a contiguous LDS range, no search) into a contiguous
+// per-wave list, then walks the list's unordered pairs in folded order, 64
+// at a time, testing the third side and evaluating each passing triplet in
+// place.  The cutoff tests use the periodic form, or plain differences where
+// the staged particles are close enough for the two to agree bit for bit.
+// The j and k gradients go to per-wave list accumulators (LDS atomics),
+// folded into the staged particles' accumulators once per owner; the
+// workgroup adds those to global memory once per staged particle that
+// received any.  The owners of a neighbourhood larger than NBCAP, and an
+// owner whose list is longer than LCAP, are queued for the dense-owner
+// kernel below instead.
 
 constexpr int CELL_WAVES = 4;
-constexpr int NBCAP = 384;
+constexpr int NBCAP = 256;
 constexpr int LCAP = 96;   // list positions fit in the 8-bit halves of a queue entry
 constexpr int RQ = 128;    // per-wave queue of passing pairs (>= 2 x 64)
 
@@ -883,6 +887,22 @@ __device__ void stage_cells(const CutDev &A, const SysGrid &sg, int c, int *cst,
     }
 }
 
+// Squared distance for the cutoff test.  PER: the periodic form
+// min(t, L - t) per component (Algorithm.cpp's cutoff test).  !PER: the plain
+// difference, which gives the same bits whenever every |difference| is below
+// L/2, because min(t, L - t) is then t itself and t * t = d * d.
+template <bool PER>
+__device__ __forceinline__ double cut_sq(double ax, double ay, double az, double bx, double by, double bz, double Lx,
+                                         double Ly, double Lz)
+{
+    if constexpr (PER) {
+        return sqr_dist_periodic_min(ax, ay, az, bx, by, bz, Lx, Ly, Lz);
+    } else {
+        const double dx = bx - ax, dy = by - ay, dz = bz - az;
+        return dx * dx + dy * dy + dz * dz;
+    }
+}
+
 // Global slot (cell order) of staged index t.
 __device__ __forceinline__ int staged_slot(const int *cst, const int *cpre, int nk, int t)
 {
@@ -914,7 +934,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     __shared__ double wls[CELL_WAVES][6][LCAP];
     __shared__ unsigned short ring[CELL_WAVES][RQ];
     __shared__ int cst[32], cpre[33], sinfo[4];
-    __shared__ int s_next;
+    __shared__ int s_next, s_far, s_wcnt[CELL_WAVES];
 
     const CutDev &A = a.c;
     const int c = cell_for_block(blockIdx.x, a.ncell);
@@ -942,7 +962,77 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     // 1. the staged cells: c first, then its forward neighbour cells
     if (wv == 0) stage_cells(A, sg, c, cst, cpre, sinfo);
     __syncthreads();
-    const int total = sinfo[3], own = sinfo[1], nk = sinfo[2];
+    const int cand = sinfo[3], own = sinfo[1], nk = sinfo[2];
+
+    // 2. stage, in order, c's particles and those particles of the forward
+    // cells that lie within the cutoff of c's box (periodic box distance,
+    // with a relative margin of 1e-6: no owner in c reaches the others), and
+    // clear their accumulators.  c's own particles come first, so owner i is
+    // staged particle i.  With at least 7 cells per dimension, staged
+    // particles whose unwrapped cell coordinates all lie within
+    // [c - 1.01, c + 2.01) are less than 3.02 cell widths (< L/2) apart in
+    // every dimension, so the cutoff tests may use plain differences
+    // (cut_sq<false>); a particle outside (a wrapped neighbour cell, or one
+    // outside the box) sends the workgroup to the periodic form.
+    if (tid == 0) {
+        s_next = 0;
+        s_far = (sg.nc[0] < 7 || sg.nc[1] < 7 || sg.nc[2] < 7) ? 1 : 0;
+    }
+    int total = 0;
+    {
+        const int lc = c - sg.cbase;
+        const double ccx = (double)(lc % sg.nc[0]), ccy = (double)((lc / sg.nc[0]) % sg.nc[1]),
+                     ccz = (double)(lc / (sg.nc[0] * sg.nc[1]));
+        const double wx = Lx / sg.nc[0], wy = Ly / sg.nc[1], wz = Lz / sg.nc[2];
+        const double ix = 1.0 / wx, iy = 1.0 / wy, iz = 1.0 / wz;
+        const double mx = sg.lo[0] + (ccx + 0.5) * wx, my = sg.lo[1] + (ccy + 0.5) * wy,
+                     mz = sg.lo[2] + (ccz + 0.5) * wz;
+        const double reach2 = c2 * (1.0 + 1e-6);
+        auto gap = [](double d, double L, double w) {
+            d -= L * __builtin_rint(d / L);
+            const double g = __builtin_fabs(d) - 0.5 * w;
+            return g > 0.0 ? g : 0.0;
+        };
+        bool far = false;
+        for (int t0 = 0; t0 < cand; t0 += CELL_WAVES * WAVE) {
+            const int t = t0 + tid;
+            bool keep = false;
+            double x = 0.0, y = 0.0, z = 0.0;
+            int q = 0;
+            if (t < cand) {
+                const int slot = staged_slot(cst, cpre, nk, t);
+                x = a.xs[slot];
+                y = a.ys[slot];
+                z = a.zs[slot];
+                q = A.cell_idx[slot];
+                const double gx = gap(x - mx, Lx, wx), gy = gap(y - my, Ly, wy), gz = gap(z - mz, Lz, wz);
+                keep = t < own || gx * gx + gy * gy + gz * gz <= reach2;
+            }
+            const unsigned long long m = __ballot(keep);
+            if (lane == 0) s_wcnt[wv] = __popcll(m);
+            __syncthreads();
+            int pos = total + lane_prefix(m);
+            for (int w = 0; w < CELL_WAVES; ++w) {
+                const int k = s_wcnt[w];
+                if (w < wv) pos += k;
+                total += k;
+            }
+            if (keep && pos < NBCAP) {
+                nx[pos] = x;
+                ny[pos] = y;
+                nz[pos] = z;
+                nid[pos] = q;
+                gxs[pos] = 0.0;
+                gys[pos] = 0.0;
+                gzs[pos] = 0.0;
+                const double ux = (x - sg.lo[0]) * ix - ccx, uy = (y - sg.lo[1]) * iy - ccy,
+                             uz = (z - sg.lo[2]) * iz - ccz;
+                far = far || !(ux >= -1.01 && ux < 2.01 && uy >= -1.01 && uy < 2.01 && uz >= -1.01 && uz < 2.01);
+            }
+            __syncthreads();  // s_wcnt is reused
+        }
+        if (far) s_far = 1;
+    }
     if (total > NBCAP) {
         // hand every owner of c to the dense-owner kernel
         if (!a.dense_on) {
@@ -954,165 +1044,163 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
         for (int t = tid; t < own; t += CELL_WAVES * WAVE) a.owners[s_next + t] = sinfo[0] + t;
         return;
     }
-
-    // 2. stage positions and clear the accumulators
-    for (int t = tid; t < total; t += CELL_WAVES * WAVE) {
-        const int slot = staged_slot(cst, cpre, nk, t);
-        nx[t] = a.xs[slot];
-        ny[t] = a.ys[slot];
-        nz[t] = a.zs[slot];
-        nid[t] = A.cell_idx[slot];
-        gxs[t] = 0.0;
-        gys[t] = 0.0;
-        gzs[t] = 0.0;
-    }
-    if (tid == 0) s_next = 0;
     __syncthreads();
 
-    // 3. owners of cell c, claimed one at a time by the waves
+    // 3. owners of cell c, claimed one at a time by the waves; the cutoff
+    // tests take the plain form unless the staging found a far particle
     unsigned short *list = wl[wv];
     double *qx = wls[wv][0], *qy = wls[wv][1], *qz = wls[wv][2];
     double *hx = wls[wv][3], *hy = wls[wv][4], *hz = wls[wv][5];
-    for (;;) {
-        int i = 0;
-        if (lane == 0) i = atomicAdd(&s_next, 1);
-        i = __builtin_amdgcn_readfirstlane(i);
-        if (i >= own) break;
-        const double xi = nx[i], yi = ny[i], zi = nz[i];
+    auto owners = [&](auto per) {
+        constexpr bool PER = decltype(per)::value;
+        for (;;) {
+            int i = 0;
+            if (lane == 0) i = atomicAdd(&s_next, 1);
+            i = __builtin_amdgcn_readfirstlane(i);
+            if (i >= own) break;
+            const double xi = nx[i], yi = ny[i], zi = nz[i];
 
-        // the owner's forward candidates within the cutoff, in staging
-        // order, copied to this wave's contiguous list
-        int len = 0;
-        for (int t0 = i + 1; t0 < total; t0 += WAVE) {
-            const int t = t0 + lane;
-            double xt = 0.0, yt = 0.0, zt = 0.0;
-            bool ok = t < total;
-            if (ok) {
-                xt = nx[t];
-                yt = ny[t];
-                zt = nz[t];
-                ok = !(sqr_dist_periodic_min(xi, yi, zi, xt, yt, zt, Lx, Ly, Lz) > c2);
-            }
-            const unsigned long long m = __ballot(ok);
-            const int pos = len + lane_prefix(m);
-            if (ok && pos < LCAP) {
-                list[pos] = (unsigned short)t;
-                qx[pos] = xt;
-                qy[pos] = yt;
-                qz[pos] = zt;
-                hx[pos] = 0.0;
-                hy[pos] = 0.0;
-                hz[pos] = 0.0;
-            }
-            len += __popcll(m);
-        }
-        if (len > LCAP) {
-            // too long for this wave's list: the dense-owner kernel takes i
-            if (lane == 0) {
-                if (a.dense_on) a.owners[atomicAdd(&A.st->dense_n, 1)] = sinfo[0] + i;
-                else want_dense_kernel(A.st);
-            }
-            continue;
-        }
-        wave_sync();
-
-        // Every unordered pair of the list once, in folded order: item
-        // t = (o - 1) len + r pairs r with (r + o) mod len, o = 1 .. H;
-        // for even len the last len / 2 items pair r with r + len / 2.
-        // Lane l takes items l, l + 64, ...; (r, o) advance by
-        // (64 mod len, 64 / len) with a carry.  Pairs that pass the test of
-        // their third side are queued (in item order, so the queue walks the
-        // list nearly in sequence) and evaluated 64 at a time.  Only the j
-        // and k gradients are accumulated: the owner's is minus their sum
-        // (the triplet's gradients sum to zero), taken from the list at the
-        // end.
-        const int npairs = len * (len - 1) / 2;
-        if (npairs > 0) {
-            unsigned short *rq = ring[wv];
-            int head = 0, qn = 0;
-            auto flush = [&](int nq) {
-                wave_sync();
-                if (lane < nq) {
-                    const unsigned e = rq[(head + lane) & (RQ - 1)];
-                    const int r = (int)(e & 0xffu), kk = (int)(e >> 8);
-                    const double xj = qx[r], yj = qy[r], zj = qz[r];
-                    const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
-                    const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
-                    const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
-                    const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
-                    const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
-                    const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
-                    const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-                    T eu, ev, ew, et;
-                    at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
-                    if (GRAD) {
-                        const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
-                        lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
-                        lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
-                        lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
-                        lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
-                        lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
-                        lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
-                    }
-                    if (WANT_E) esum += (double)et;
-                    ++cnt;
-                }
-                wave_sync();
-                head = (head + nq) & (RQ - 1);
-                qn -= nq;
-            };
-            const int q64 = WAVE / len, r64 = WAVE - q64 * len;
-            int o = lane / len;
-            int r = lane - o * len;
-            ++o;
-            for (int t0 = 0; t0 < npairs; t0 += WAVE) {
-                bool ok = false;
-                int kk = 0;
-                if (t0 + lane < npairs) {
-                    kk = r + o;
-                    kk = kk >= len ? kk - len : kk;
-                    ok = !(sqr_dist_periodic_min(qx[r], qy[r], qz[r], qx[kk], qy[kk], qz[kk], Lx, Ly, Lz) > c2);
+            // the owner's forward candidates within the cutoff, in staging
+            // order, copied to this wave's contiguous list
+            int len = 0;
+            for (int t0 = i + 1; t0 < total; t0 += WAVE) {
+                const int t = t0 + lane;
+                double xt = 0.0, yt = 0.0, zt = 0.0;
+                bool ok = t < total;
+                if (ok) {
+                    xt = nx[t];
+                    yt = ny[t];
+                    zt = nz[t];
+                    ok = !(cut_sq<PER>(xi, yi, zi, xt, yt, zt, Lx, Ly, Lz) > c2);
                 }
                 const unsigned long long m = __ballot(ok);
-                if (ok) rq[(head + qn + lane_prefix(m)) & (RQ - 1)] = (unsigned short)(r | (kk << 8));
-                qn += __popcll(m);
-                if (qn >= WAVE) flush(WAVE);
-                r += r64;
-                o += q64;
-                if (r >= len) {
-                    r -= len;
-                    ++o;
+                const int pos = len + lane_prefix(m);
+                if (ok && pos < LCAP) {
+                    list[pos] = (unsigned short)t;
+                    qx[pos] = xt;
+                    qy[pos] = yt;
+                    qz[pos] = zt;
+                    hx[pos] = 0.0;
+                    hy[pos] = 0.0;
+                    hz[pos] = 0.0;
+                }
+                len += __popcll(m);
+            }
+            if (len > LCAP) {
+                // too long for this wave's list: the dense-owner kernel takes i
+                if (lane == 0) {
+                    if (a.dense_on) a.owners[atomicAdd(&A.st->dense_n, 1)] = sinfo[0] + i;
+                    else want_dense_kernel(A.st);
+                }
+                continue;
+            }
+            wave_sync();
+
+            // Every unordered pair of the list once, in folded order: item
+            // t = (o - 1) len + r pairs r with (r + o) mod len, o = 1 .. H;
+            // for even len the last len / 2 items pair r with r + len / 2.
+            // Lane l takes items l, l + 64, ...; (r, o) advance by
+            // (64 mod len, 64 / len) with a carry.  Pairs that pass the test of
+            // their third side are queued (in item order, so the queue walks the
+            // list nearly in sequence) and evaluated 64 at a time.  Only the j
+            // and k gradients are accumulated: the owner's is minus their sum
+            // (the triplet's gradients sum to zero), taken from the list at the
+            // end.
+            const int npairs = len * (len - 1) / 2;
+            if (npairs > 0) {
+                unsigned short *rq = ring[wv];
+                int head = 0, qn = 0;
+                auto flush = [&](int nq) {
+                    wave_sync();
+                    if (lane < nq) {
+                        const unsigned e = rq[(head + lane) & (RQ - 1)];
+                        const int r = (int)(e & 0xffu), kk = (int)(e >> 8);
+                        const double xj = qx[r], yj = qy[r], zj = qz[r];
+                        const double xk = qx[kk], yk = qy[kk], zk = qz[kk];
+                        const double djx = xj - xi, djy = yj - yi, djz = zj - zi;
+                        const double dkx = xk - xi, dky = yk - yi, dkz = zk - zi;
+                        const double ex = xk - xj, ey = yk - yj, ez = zk - zj;
+                        const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
+                        const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
+                        const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+                        T eu, ev, ew, et;
+                        at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
+                        if (GRAD) {
+                            const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
+                            lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
+                            lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
+                            lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
+                            lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
+                            lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
+                            lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
+                        }
+                        if (WANT_E) esum += (double)et;
+                        ++cnt;
+                    }
+                    wave_sync();
+                    head = (head + nq) & (RQ - 1);
+                    qn -= nq;
+                };
+                // 64 / len and lane / len from one FP32 reciprocal: for
+                // len <= LCAP the fraction of (k + 0.5) / len stays 0.5 / len
+                // away from an integer, far beyond the FP32 rounding
+                const float inv = 1.0f / (float)len;
+                const int q64 = (int)(64.5f * inv), r64 = WAVE - q64 * len;
+                int o = (int)(((float)lane + 0.5f) * inv);
+                int r = lane - o * len;
+                ++o;
+                auto test = [&](bool live) {
+                    int kk = r + o;
+                    kk = kk >= len ? kk - len : kk;
+                    kk = live ? kk : r;  // lanes past the last item read their own entry
+                    const bool ok = live && !(cut_sq<PER>(qx[r], qy[r], qz[r], qx[kk], qy[kk], qz[kk], Lx, Ly, Lz) > c2);
+                    const unsigned long long m = __ballot(ok);
+                    if (ok) rq[(head + qn + lane_prefix(m)) & (RQ - 1)] = (unsigned short)(r | (kk << 8));
+                    qn += __popcll(m);
+                    if (qn >= WAVE) flush(WAVE);
+                    r += r64;
+                    o += q64;
+                    if (r >= len) {
+                        r -= len;
+                        ++o;
+                    }
+                };
+                // whole rounds of 64 items, then the last partial round
+                const int full = npairs & ~(WAVE - 1);
+                for (int t0 = 0; t0 < full; t0 += WAVE) test(true);
+                if (full < npairs) test(full + lane < npairs);
+                if (qn > 0) flush(qn);
+            }
+            if (GRAD) {
+                // the list's gradients go to the staged accumulators; the owner's
+                // is minus their sum (16-lane row sums in the VALU, then one LDS
+                // atomic per row)
+                wave_sync();
+                double gix = 0.0, giy = 0.0, giz = 0.0;
+                for (int t = lane; t < len; t += WAVE) {
+                    const int q = list[t];
+                    const double hxv = hx[t], hyv = hy[t], hzv = hz[t];
+                    lds_add(&gxs[q], hxv);
+                    lds_add(&gys[q], hyv);
+                    lds_add(&gzs[q], hzv);
+                    gix -= hxv;
+                    giy -= hyv;
+                    giz -= hzv;
+                }
+                gix = row_sum_last(gix);
+                giy = row_sum_last(giy);
+                giz = row_sum_last(giz);
+                if ((lane & 15) == 15) {
+                    lds_add(&gxs[i], gix);
+                    lds_add(&gys[i], giy);
+                    lds_add(&gzs[i], giz);
                 }
             }
-            if (qn > 0) flush(qn);
-        }
-        if (GRAD) {
-            // the list's gradients go to the staged accumulators; the owner's
-            // is minus their sum (16-lane row sums in the VALU, then one LDS
-            // atomic per row)
             wave_sync();
-            double gix = 0.0, giy = 0.0, giz = 0.0;
-            for (int t = lane; t < len; t += WAVE) {
-                const int q = list[t];
-                const double hxv = hx[t], hyv = hy[t], hzv = hz[t];
-                lds_add(&gxs[q], hxv);
-                lds_add(&gys[q], hyv);
-                lds_add(&gzs[q], hzv);
-                gix -= hxv;
-                giy -= hyv;
-                giz -= hzv;
-            }
-            gix = row_sum_last(gix);
-            giy = row_sum_last(giy);
-            giz = row_sum_last(giz);
-            if ((lane & 15) == 15) {
-                lds_add(&gxs[i], gix);
-                lds_add(&gys[i], giy);
-                lds_add(&gzs[i], giz);
-            }
         }
-        wave_sync();
-    }
+    };
+    if (s_far) owners(std::true_type{});
+    else owners(std::false_type{});
     __syncthreads();
 
     // 4. add the staged particles' gradients to global memory.  Staged
