@@ -487,10 +487,13 @@ bool grow_after_overflow(tb3_ctx *c)
     return true;
 }
 
-void accumulate_profile(tb3_ctx *c, unsigned long long triplets)
+// Phase times of one pass: ev[0..2] bracket the status reset and the cell
+// list (read once the first attempt has finished), ev[2..4] the sweep and
+// the update of the attempt that produced the result.
+void accumulate_profile(tb3_ctx *c, unsigned long long triplets, const float *prep)
 {
-    float ms[4] = {0, 0, 0, 0};
-    for (int k = 0; k < 4; ++k) HIPCHK(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
+    float ms[4] = {prep[0], prep[1], 0, 0};
+    for (int k = 2; k < 4; ++k) HIPCHK(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
     c->prof_acc[0] += 1.0;
     for (int k = 0; k < 4; ++k) c->prof_acc[1 + k] += ms[k];
     c->prof_acc[5] += (double)triplets;
@@ -504,7 +507,6 @@ void finish_overflowed(tb3_ctx *c, hipStream_t s, DeviceState &d, bool integrate
     const bool exact = c->precision == TB3_FP64_EXACT;
     for (int retry = 0; grow_after_overflow(c); ++retry) {
         if (retry >= 8) throw Fail(TB3_EHIP, "the cutoff sweep kept overflowing its LDS staging");
-        c->recording = false;
         launch_reset_status(s, c->d_status, 1);  // keeps the non-finite flag of the binning pass
         if (grad && !exact) {
             launch_zero(s, d.fx, d.n);
@@ -513,6 +515,7 @@ void finish_overflowed(tb3_ctx *c, hipStream_t s, DeviceState &d, bool integrate
         }
         launch_sweep(c, s, d, grad, energy, !integrate);
         if (integrate) launch_tail(c, s, d, dt, g, grad, false);
+        if (c->recording) HIPCHK(hipEventRecord(c->ev[4], s));
         read_status(c, s);
     }
 }
@@ -532,8 +535,11 @@ unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool inte
     if (integrate) launch_tail(c, s, d, dt, g, grad, false);
     if (c->recording) HIPCHK(hipEventRecord(c->ev[4], s));
     read_status(c, s);
+    float prep[2] = {0.0f, 0.0f};
+    if (c->recording)
+        for (int k = 0; k < 2; ++k) HIPCHK(hipEventElapsedTime(&prep[k], c->ev[k], c->ev[k + 1]));
     finish_overflowed(c, s, d, integrate, dt, g, grad, energy);
-    if (c->recording) accumulate_profile(c, c->h_status->count[0]);
+    if (c->recording) accumulate_profile(c, c->h_status->count[0], prep);
     return c->h_status->count[0];
 }
 
