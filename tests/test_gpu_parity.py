@@ -8,7 +8,7 @@ Bars (written into each test):
     per-particle |df| <= 1e-12 max(|f|, rms|f|) and RMS of |df|/|f| <= 1e-12;
     counts bit-exact.  (A particle whose net force nearly cancels amplifies
     ulp-level differences in the pure ratio |df|/|f|, which is held to 1e-10.)
-  * TB3_FP32_MIXED: one sweep, |df| <= 1e-5 max(|f|, rms|f|), RMS |df|/|f|
+  * TB3_FP32_MIXED: one sweep, |df| <= 1e-6 max(|f|, rms|f|), RMS |df|/|f|
     <= 1e-5; counts bit-exact (the cutoff test is FP64).  bench.py states the
     measured errors.
 """
@@ -170,7 +170,7 @@ def test_fast_sweep_particles_outside_box(eng, oracle, precision):
     if precision == "fp64":
         assert e["max_rel_floored"] <= 1e-12 and e["rms_rel"] <= 1e-10
     else:
-        assert e["max_rel_floored"] <= 1e-5 and e["rms_rel"] <= 1e-3
+        assert e["max_rel_floored"] <= 1e-6 and e["rms_rel"] <= 1e-3
 
 
 @pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0),
@@ -184,10 +184,12 @@ def test_fp32_mixed_one_sweep(eng, oracle, algo, n, box, cutoff):
     assert cnt == co
     e = force_errors(f, fo)
     say(f"  fp32 errors {e}")
-    # FP32 rounding of u, v, w (~6e-8 each) bounds each triplet to ~1e-7;
-    # particles whose net force nearly cancels amplify that in the pure
-    # per-particle ratio, so the floored ratio (floor = RMS |F|) carries the bar.
-    assert e["max_rel_floored"] <= 1e-5
+    # FP32 rounding of u, v, w (~6e-8 each) bounds each triplet to ~1e-7
+    # (close pairs included: their third particle's coefficient sum is formed
+    # without cancellation); particles whose net force nearly cancels amplify
+    # that in the pure per-particle ratio, so the floored ratio (floor = RMS
+    # |F|) carries the 1e-6 bar.
+    assert e["max_rel_floored"] <= 1e-6
     assert e["rms_rel"] <= 1e-5
     assert e["max_rel"] <= 1e-3
 
@@ -215,7 +217,7 @@ def test_energy_and_adjoint(eng, oracle, precision):
         e = force_errors(g, fo)
         say(f"  {precision} adjoint errors {e}")
         if precision == "fp32":
-            assert e["max_rel_floored"] <= 1e-5 and e["rms_rel"] <= 1e-5
+            assert e["max_rel_floored"] <= 1e-6 and e["rms_rel"] <= 1e-5
         else:
             assert e["max_rel_floored"] <= 1e-12 and e["rms_rel"] <= 1e-12
             assert e["max_rel"] <= 1e-10

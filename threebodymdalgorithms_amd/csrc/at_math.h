@@ -175,9 +175,19 @@ __device__ __forceinline__ float rsqrt_t(float x) { return rsqrt_f32(x); }
 // C = -u+v+w are formed in FP64 before rounding to FP32, because for
 // elongated triangles (two long sides of nearly equal length) the difference
 // cancels and FP32 inputs would leave only a few correct bits; everything
-// after that is FP32.
-__device__ __forceinline__ void at_coeffs_mixed(double u64, double v64, double w64, float nu, float K, float &eu,
-                                                float &ev, float &ew, float &e, bool need_w)
+// after that is FP32, and the coefficients come back in FP64.
+//
+// A triplet with one short side (a close pair) gives its third particle two
+// huge, nearly opposite terms, -(eu dij + ev dik) for particle i when w is
+// the short side: the FP32 rounding of eu and ev alone would leave that
+// gradient with ~1e-6 relative error.  Two exact identities avoid this:
+// X + Y = 2uC, X + Z = 2vB and Y + Z = 2wA give S = X + Y + Z and the third
+// particle's coefficient sum (eu + ev for w, ev + ew for u, eu + ew for v)
+// without cancellation; the coefficient paired with the sum is then rebuilt
+// as sum - partner in FP64 (exact for FP32 operands), so that the third
+// particle sees sum * (its vector) plus partner * (the short side).
+__device__ __forceinline__ void at_coeffs_mixed(double u64, double v64, double w64, float nu, float K, double &cu,
+                                                double &cv, double &cw, float &e, bool need_w)
 {
     const float u = (float)u64, v = (float)v64, w = (float)w64;
     const float A = (float)((u64 + v64) - w64);
@@ -189,19 +199,43 @@ __device__ __forceinline__ void at_coeffs_mixed(double u64, double v64, double w
     const float X = B * C, Y = A * C, Z = A * B;
     const float Q = (A * X) * s2;
     const float R = __builtin_fmaf(-2.5f * K, Q, -1.5f * nu);
-    const float S = (X + Y) + Z;
+    // the shortest side m, the sum o of the other two, the factor P of 2 m P
+    // in S, and the product G that completes S = G + 2 m P
+    const bool wmin = w64 <= u64 && w64 <= v64;
+    const bool umin = !wmin && u64 <= v64;
+    const float m = wmin ? w : (umin ? u : v);
+    const float o = wmin ? u + v : (umin ? v + w : w + u);
+    const float P = wmin ? A : (umin ? C : B);
+    const float G = wmin ? X : (umin ? Z : Y);
+    const float S = __builtin_fmaf(2.0f * m, P, G);
     const float KS = K * S;
     const float m2K = -2.0f * K;
     const float c2s5 = 2.0f * (s2 * s2 * s);
-    eu = c2s5 * __builtin_fmaf(R, v * w, __builtin_fmaf(m2K, Z, KS));
-    ev = c2s5 * __builtin_fmaf(R, u * w, __builtin_fmaf(m2K, Y, KS));
-    ew = need_w ? c2s5 * __builtin_fmaf(R, uv, __builtin_fmaf(m2K, X, KS)) : 0.0f;
+    const double du = (double)(c2s5 * __builtin_fmaf(R, v * w, __builtin_fmaf(m2K, Z, KS)));
+    const double dv = (double)(c2s5 * __builtin_fmaf(R, u * w, __builtin_fmaf(m2K, Y, KS)));
     e = (s2 * s) * __builtin_fmaf(K, Q, nu);
+    if (need_w) {
+        const double dw = (double)(c2s5 * __builtin_fmaf(R, uv, __builtin_fmaf(m2K, X, KS)));
+        const double sum = (double)(c2s5 * __builtin_fmaf(R, m * o, __builtin_fmaf(m2K, 2.0f * m * P, 2.0f * KS)));
+        // w shortest, third particle i: -(cu dij + cv dik), rebuild cv;
+        // u shortest, k: cv dik + cw djk, rebuild cw; v shortest, j:
+        // cu dij - cw djk with djk ~ -dij, rebuild cw
+        const double rebuilt = sum - (umin ? dv : du);
+        cu = du;
+        cv = wmin ? rebuilt : dv;
+        cw = wmin ? dw : rebuilt;
+    } else {
+        cu = du;
+        cv = dv;
+        cw = 0.0;
+    }
 }
 
-// Dispatch on the per-triplet type: FP64 throughout, or the mixed form.
+// Dispatch on the per-triplet type: FP64 throughout, or the mixed form; the
+// gradient coefficients come back in FP64 either way.
 template <class T, bool NEED_W>
-__device__ __forceinline__ void at_coeffs_from64(double u, double v, double w, T nu, T K, T &eu, T &ev, T &ew, T &e);
+__device__ __forceinline__ void at_coeffs_from64(double u, double v, double w, T nu, T K, double &cu, double &cv,
+                                                 double &cw, T &e);
 
 // Gradient coefficients of one triplet in the squared-distance form.
 // Returns eu = 2 dE/du, ev = 2 dE/dv, ew = 2 dE/dw and the energy e, so that
@@ -231,10 +265,11 @@ __device__ __forceinline__ void at_coeffs(T u, T v, T w, T nu, T K, T &eu, T &ev
 }
 
 template <class T, bool NEED_W>
-__device__ __forceinline__ void at_coeffs_from64(double u, double v, double w, T nu, T K, T &eu, T &ev, T &ew, T &e)
+__device__ __forceinline__ void at_coeffs_from64(double u, double v, double w, T nu, T K, double &cu, double &cv,
+                                                 double &cw, T &e)
 {
-    if constexpr (sizeof(T) == 8) at_coeffs<double, NEED_W>(u, v, w, nu, K, eu, ev, ew, e);
-    else at_coeffs_mixed(u, v, w, nu, K, eu, ev, ew, e, NEED_W);
+    if constexpr (sizeof(T) == 8) at_coeffs<double, NEED_W>(u, v, w, nu, K, cu, cv, cw, e);
+    else at_coeffs_mixed(u, v, w, nu, K, cu, cv, cw, e, NEED_W);
 }
 
 }  // namespace tb3

@@ -693,10 +693,10 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
             const double u = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
             const double v = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
             const double w = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-            T eu, ev, ew, et;
-            at_coeffs_from64<T, GRAD>(u, v, w, nu, K, eu, ev, ew, et);
+            double cu, cv, cw;
+            T et;
+            at_coeffs_from64<T, GRAD>(u, v, w, nu, K, cu, cv, cw, et);
             if (GRAD) {
-                const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
                 gpx = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gpx));
                 gpy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, gpy));
                 gpz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, gpz));
@@ -1123,10 +1123,10 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
                         const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
                         const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
                         const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-                        T eu, ev, ew, et;
-                        at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
+                        double cu, cv, cw;
+                        T et;
+                        at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, cu, cv, cw, et);
                         if (GRAD) {
-                            const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
                             lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
                             lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
                             lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
@@ -1352,10 +1352,10 @@ __global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
                     const double u64 = __builtin_fma(djx, djx, __builtin_fma(djy, djy, djz * djz));
                     const double v64 = __builtin_fma(dkx, dkx, __builtin_fma(dky, dky, dkz * dkz));
                     const double w64 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
-                    T eu, ev, ew, et;
-                    at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, eu, ev, ew, et);
+                    double cu, cv, cw;
+                    T et;
+                    at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, cu, cv, cw, et);
                     if (GRAD) {
-                        const double cu = (double)eu, cv = (double)ev, cw = (double)ew;
                         gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
                         giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
                         giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
@@ -1878,11 +1878,12 @@ __global__ __launch_bounds__(WAVE) void k_forces_tiles(TileDev a)
             const double djkx = xk - xj, djky = yk - yj, djkz = zk - zj;
             const double v = __builtin_fma(dikx, dikx, __builtin_fma(diky, diky, dikz * dikz));
             const double w = __builtin_fma(djkx, djkx, __builtin_fma(djky, djky, djkz * djkz));
-            T eu, ev, ew, et;
-            at_coeffs_from64<T, true>(u64, v, w, nu, K, eu, ev, ew, et);
-            const double cu = ok ? (double)eu : 0.0;
-            const double cv = ok ? (double)ev : 0.0;
-            const double cw = ok ? (double)ew : 0.0;
+            double cu, cv, cw;
+            T et;
+            at_coeffs_from64<T, true>(u64, v, w, nu, K, cu, cv, cw, et);
+            cu = ok ? cu : 0.0;
+            cv = ok ? cv : 0.0;
+            cw = ok ? cw : 0.0;
             if (GRAD) {
                 gix = __builtin_fma(-cu, dijx, __builtin_fma(-cv, dikx, gix));
                 giy = __builtin_fma(-cu, dijy, __builtin_fma(-cv, diky, giy));
