@@ -53,3 +53,48 @@ def test_cli_matches_oracle_cli(tmp_path, algo, n, box, cutoff, steps, full):
     ours = re.search(r"have done (\d+) particle interactions", g.stdout)
     ref = re.search(r"step 0: (\d+) particle interactions", o.stdout)
     assert ours and ref and ours.group(1) == ref.group(1)
+
+
+@pytest.mark.parametrize("algo,n,box,cutoff,steps", [("p3bca", 600, 9.0, 2.0, 3), ("auta", 120, 4.0, 0.0, 2)])
+def test_cli_profile_json(tmp_path, algo, n, box, cutoff, steps):
+    """-op writes the PROFILE_3BMDA JSON (main.cpp:127-451) at one process:
+    per-step sweep times under "calculateForces" and "calculateInteractions",
+    the hitrate with the reference's averaging (step 0's accepted/candidate
+    ratio over the number of recorded hitrates), one "0" member per step in
+    "hitrate per proc", and a zero "stvr"."""
+    import json
+    import re
+    src = tmp_path / "particles.csv"
+    generators.main([str(v) for v in ("--generator", "uniform", "--numparticles", n, "--seed0", 77, "--blx", box,
+                                       "--bly", box, "--blz", box, "--mass", 1, "--output", src)])
+    common = ["-a", algo, "-i", steps, "-d", 0.001, "-gy", -9.81, "-c", cutoff, "-csv", src]
+    say(f"cli profile json {algo} n={n} steps={steps}")
+    prof = tmp_path / "profile.json"
+    g = run(tb3.CLI_PATH, *common, "-op", prof)
+    assert g.returncode == 0, g.stderr
+    o = run(ORACLE_CLI, *common, "-o", tmp_path / "ref.csv", "--counts")
+    assert o.returncode == 0, o.stderr
+    counts = [int(c) for c in re.findall(r"step \d+: (\d+) particle interactions", o.stdout)]
+    assert len(counts) == steps
+    doc = json.loads(prof.read_text(), object_pairs_hook=lambda pairs: pairs)
+    assert [k for k, _ in doc] == ["times", "hitrate", "hitrate per proc", "stvr"]
+    d = dict(doc)
+    times = dict(d["times"])
+    assert list(times) == ["calculateForces", "calculateInteractions"]
+    for key in times:
+        t = dict(times[key])
+        assert [k for k, _ in times[key]] == ["time unit", "avg time all proc", "total time all proc",
+                                               "total avg time union proc", "total max time union proc",
+                                               "times per proc"]
+        per = dict(t["times per proc"])["0"]
+        assert len(per) == steps and all(isinstance(v, int) and v > 0 for v in per)
+        assert t["time unit"] == "nanosecond"
+        assert t["total max time union proc"] == sum(per)
+        assert t["total time all proc"] == float(sum(per)) == t["total avg time union proc"]
+        assert t["avg time all proc"] == pytest.approx(sum(per) / steps, rel=1e-15)
+    cand = float(n) * float(n - 1) * float(n - 2) / 6.0
+    hr = [c / cand for c in counts]
+    assert dict(d["hitrate"])["avg hitrate per sim step"] == [pytest.approx(hr[0] / steps, rel=1e-15)]
+    assert [k for k, _ in d["hitrate per proc"]] == ["0"] * steps
+    assert [v for _, v in d["hitrate per proc"]] == [pytest.approx(h, rel=1e-15) for h in hr]
+    assert [dict(x) for x in d["stvr"]] == [{"rank": [0], "stvr": 0.0}]

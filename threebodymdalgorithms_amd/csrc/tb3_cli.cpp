@@ -2,9 +2,9 @@
 //
 // Same flags as Utility::cliParse (utility/cli.hpp:47-187): -a/--algorithm
 // (nata|p3bca|auta), -i/--iterations, -d/--delta, -gx/-gy/-gz
-// (--gravityX/Y/Z), -c/--cutoff, -csv/--csv, -o/--out; -decomp, -op and
-// -benchyaml are accepted and ignored (they only steer multi-process runs and
-// the profile dump).  Unknown flags are reported and skipped (cli.hpp:171-178);
+// (--gravityX/Y/Z), -c/--cutoff, -csv/--csv, -o/--out, -op/--outProfile;
+// -decomp and -benchyaml are accepted and ignored (they only steer
+// multi-process runs).  Unknown flags are reported and skipped (cli.hpp:171-178);
 // a flag without its value or an unparsable value prints the help and exits 1
 // (cli.hpp:57-60, 181-184).  Defaults are the reference's zero-initialised
 // arguments (main.cpp:28): NATA, 0 iterations, dt 0, cutoff 0, g = 0.
@@ -27,7 +27,19 @@
 //                                 (the MEASURESIMSTEP_3BMDA line, Simulation.cpp:50-72)
 //   --counts                      print the interaction counts of step 0 as the
 //                                 reference's VLEVEL line (main.cpp:561-569)
+//
+// -op <file> writes the profile JSON of a PROFILE_3BMDA build of the reference
+// (main.cpp:127-451) at one process, for the keys this path has: per-step
+// "calculateInteractions" and "calculateForces" times in ns (the sweep kernel's
+// device time: on the GPU the potential is evaluated inside the sweep), the
+// hitrate (accepted / candidate triplets, Algorithm.cpp:92-138 counters, with
+// the reference's averaging: step 0's value over the number of recorded
+// hitrates, main.cpp:403-416, and one "0" member per step in
+// "hitrate per proc") and "stvr" (0 at one process).  Doubles are written as
+// rapidjson's Writer does: shortest round-trip digits, ".0" on integral values.
 #include <algorithm>
+#include <charconv>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -52,6 +64,7 @@ struct Args {
     int precision = TB3_FP64_EXACT;
     int device = 0;
     bool fullPrecision = false, timing = false, counts = false;
+    std::string outputProfile;
 };
 
 void printHelp()
@@ -115,7 +128,9 @@ Args parse(const std::vector<std::string> &args)
                     a.inputCSV = value();
                 } else if (flag == "o" || flag == "-out") {
                     a.outputCSV = value();
-                } else if (flag == "op" || flag == "-outProfile" || flag == "benchyaml" || flag == "-benchyaml") {
+                } else if (flag == "op" || flag == "-outProfile") {
+                    a.outputProfile = value();
+                } else if (flag == "benchyaml" || flag == "-benchyaml") {
                     (void)value();
                 } else if (flag == "gx" || flag == "-gravityX") {
                     a.g[0] = std::stod(value());
@@ -227,6 +242,79 @@ bool writeCSV(const std::string &file, const Particles &t, bool full)
     return true;
 }
 
+// A double as rapidjson's Writer prints it (Grisu2 shortest digits, then
+// Prettify with maxDecimalPlaces 324): integral values keep ".0", decimal
+// notation while the decimal point falls within 21 digits and down to 1e-6,
+// exponents without padding otherwise.
+std::string jsonDouble(double v)
+{
+    if (v == 0.0) return std::signbit(v) ? "-0.0" : "0.0";
+    char buf[64];
+    const auto r = std::to_chars(buf, buf + sizeof(buf), v, std::chars_format::scientific);
+    std::string sci(buf, r.ptr);  // [-]d[.ddd]e(+|-)XX
+    std::string sign;
+    if (sci[0] == '-') {
+        sign = "-";
+        sci = sci.substr(1);
+    }
+    const size_t e = sci.find('e');
+    std::string digits = sci.substr(0, e);
+    digits.erase(std::remove(digits.begin(), digits.end(), '.'), digits.end());
+    const int exp10 = std::stoi(sci.substr(e + 1));  // value = d.ddd x 10^exp10
+    const int length = (int)digits.size();
+    const int k = exp10 - (length - 1);  // value = digits x 10^k
+    const int kk = length + k;           // position of the decimal point
+    std::string out;
+    if (k >= 0 && kk <= 21) {
+        out = digits + std::string(k, '0') + ".0";
+    } else if (0 < kk && kk <= 21) {
+        out = digits.substr(0, kk) + "." + digits.substr(kk);
+    } else if (-6 < kk && kk <= 0) {
+        out = "0." + std::string(-kk, '0') + digits;
+    } else {
+        const int x = kk - 1;
+        out = digits.substr(0, 1);
+        if (length > 1) out += "." + digits.substr(1);
+        out += "e" + std::to_string(x);
+    }
+    return sign + out;
+}
+
+// The profile JSON (see the header comment) for one process.
+bool writeProfile(const std::string &path, const std::vector<long long> &sweepNs,
+                  const std::vector<double> &hitrates)
+{
+    std::ostringstream o;
+    auto times = [&](const char *key) {
+        long long total = 0;
+        for (long long t : sweepNs) total += t;
+        const double steps = (double)sweepNs.size();
+        o << "\"" << key << "\":{\"time unit\":\"nanosecond\",\"avg time all proc\":"
+          << jsonDouble(steps > 0 ? (double)total / steps : 0.0) << ",\"total time all proc\":"
+          << jsonDouble((double)total) << ",\"total avg time union proc\":" << jsonDouble((double)total)
+          << ",\"total max time union proc\":" << total << ",\"times per proc\":{\"0\":[";
+        for (size_t i = 0; i < sweepNs.size(); ++i) o << (i ? "," : "") << sweepNs[i];
+        o << "]}}";
+    };
+    o << "{\"times\":{";
+    if (!sweepNs.empty()) {
+        times("calculateForces");
+        o << ",";
+        times("calculateInteractions");
+    }
+    o << "},\"hitrate\":{";
+    if (!hitrates.empty()) o << "\"avg hitrate per sim step\":[" << jsonDouble(hitrates[0] / (double)hitrates.size()) << "]";
+    o << "},\"hitrate per proc\":{";
+    for (size_t j = 0; j < hitrates.size(); ++j) o << (j ? "," : "") << "\"0\":" << jsonDouble(hitrates[j]);
+    o << "},\"stvr\":[";
+    if (!sweepNs.empty()) o << "{\"rank\":[0],\"stvr\":0.0}";
+    o << "]}";
+    std::ofstream f(path);
+    if (!f) return false;
+    f << o.str();
+    return (bool)f;
+}
+
 int die(tb3_ctx *ctx, const char *what, tb3_status st)
 {
     std::cerr << "tb3: " << what << " failed (" << (int)st << "): " << tb3_last_error(ctx) << std::endl;
@@ -258,12 +346,31 @@ int main(int argc, char **argv)
 
     const std::string stem = a.outputCSV.substr(0, a.outputCSV.find_last_of('.'));
     std::vector<uint64_t> counts;
+    const bool profile = !a.outputProfile.empty();
+    std::vector<long long> sweepNs;
+    std::vector<double> hitrates;
+    // candidate triplets of the one-process i<j<k sweep (Algorithm.cpp:72-92)
+    const double candidates = n >= 3 ? (double)n * (double)(n - 1) * (double)(n - 2) / 6.0 : 0.0;
+    double sweepMs = 0.0;
+    if (profile) {
+        st = tb3_set_profiling(ctx, 1);
+        if (st != TB3_OK) return die(ctx, "tb3_set_profiling", st);
+    }
     for (int i = 0; i < a.iterations; ++i) {
         const auto start = std::chrono::steady_clock::now();
         uint64_t nb = 0, nt = 0;
         st = tb3_step(ctx, a.deltaT, a.g, &nb, &nt);
         if (st != TB3_OK) return die(ctx, "tb3_step", st);
         counts.push_back(nt);
+        if (profile) {
+            double prof[6] = {0, 0, 0, 0, 0, 0};
+            st = tb3_get_profile(ctx, prof, 6);
+            if (st != TB3_OK) return die(ctx, "tb3_get_profile", st);
+            sweepNs.push_back(std::llround((prof[3] - sweepMs) * 1e6));
+            sweepMs = prof[3];
+            // the reference records a hitrate only when there were candidates
+            if (candidates > 0) hitrates.push_back((double)nt / candidates);
+        }
         if (a.timing) {
             const long long ns = std::chrono::duration_cast<std::chrono::nanoseconds>(
                                      std::chrono::steady_clock::now() - start)
@@ -280,6 +387,11 @@ int main(int argc, char **argv)
                 return 1;
             }
         }
+    }
+    if (profile && !writeProfile(a.outputProfile, sweepNs, hitrates)) {
+        std::cerr << "tb3: cannot write '" << a.outputProfile << "'" << std::endl;
+        tb3_destroy(ctx);
+        return 1;
     }
     if (a.counts)
         std::cout << "I'm proc 0 and have done " << (counts.empty() ? 0 : counts[0])
