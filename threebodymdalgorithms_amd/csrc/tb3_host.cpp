@@ -123,7 +123,6 @@ struct tb3_ctx {
     int list_cap = 128;    // LDS neighbour-list capacity per wave (power of two)
     bool cells_ok = true;  // the cell-per-workgroup kernel staged every neighbourhood so far
     bool dense_on = false;  // the dense-owner kernel follows the cell kernel (once some owner needed it)
-    int cells_grid = 1;    // CUs x resident cell-kernel workgroups per CU (reported by the tuner)
     int dense_grid = 1;    // persistent grid of the dense-owner kernel
     int ncu = 1;           // compute units of the device
     // tuned by tb3_autotune: cell width / cutoff, resident cell-kernel workgroups per CU
@@ -403,8 +402,7 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
             const int slot = (exact && grad) ? 1 : 0;  // the exact pass owns count[0]
             if (cut && c->cells_ok) {
                 launch_p3bca_cells(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
-                                   energy ? 1 : 0, std::min(c->cells_grid, c->grid.ncell), c->d_owners,
-                                   c->dense_on ? 1 : 0);
+                                   energy ? 1 : 0, c->d_owners, c->dense_on ? 1 : 0);
                 if (c->dense_on)
                     launch_p3bca_dense(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
                                        energy ? 1 : 0, c->d_owners, c->dense_grid);
@@ -732,7 +730,6 @@ tb3_status tb3_create(int device, tb3_ctx **out)
         int ncu = 0;
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
         c->ncu = std::max(1, ncu);
-        c->cells_grid = std::max(1, ncu) * p3bca_cells_blocks_per_cu();
         p3bca_dense_setup();
         c->dense_grid = std::max(1, ncu) * p3bca_dense_blocks_per_cu();
         dev_alloc(c->d_slots, 1);

@@ -69,8 +69,7 @@ struct Status {
                                   // when above half the capacity), [2] list overflow (the update
                                   // kernel skips itself), [3] the cell kernel could not stage a
                                   // neighbourhood (re-run with the per-owner kernel)
-    int queue[2];                 // cell queue of the cell kernel; returns to zero after each launch
-    int dqueue[2];                // owner queue of the dense-owner kernel; returns to zero likewise
+    int dqueue[2];                // owner queue of the dense-owner kernel; returns to zero after each launch
     int dense_n;                  // owners the cell kernel handed to the dense-owner kernel
     int want_dense;               // the cell kernel had owners for a dense-owner kernel that was not launched
     Slots *slots;                 // partial sums of the sweep kernels
@@ -134,15 +133,14 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
                        int cap, int mixed, int grad, Status *st, int count_slot, int want_energy);
 void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
                         int cap, Status *st);
-// Fast cutoff sweep with one workgroup per cell (`grid` is unused).  With
+// Fast cutoff sweep with one workgroup per cell.  With
 // dense_on, owners it cannot stage go to `owners` for launch_p3bca_dense,
 // which must follow on the same stream; without, such an owner sets
 // st->want_dense and st->flags[2] (the caller re-runs the pass with the
 // dense-owner kernel).  Non-finite positions set st->flags[2] and [3]; the
 // caller then re-runs the sweep with launch_p3bca_fast.
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid, int *owners,
-                        int dense_on);
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int dense_on);
 // Workgroups of launch_p3bca_cells that fit on one CU at once.
 int p3bca_cells_blocks_per_cu();
 // The cell kernel hands owners whose neighbourhood does not fit its LDS

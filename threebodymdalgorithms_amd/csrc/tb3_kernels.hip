@@ -2122,10 +2122,8 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
 }
 
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int grid, int *owners,
-                        int dense_on)
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int dense_on)
 {
-    (void)grid;
     if (d.n <= 0) return;
     CellsDev a;
     a.dense_on = dense_on;
