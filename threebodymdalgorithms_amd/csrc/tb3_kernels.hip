@@ -134,13 +134,16 @@ __global__ void k_rows_from_soa(int n, const double *__restrict__ x, const doubl
     rows[3 * (long long)p + 2] = z[p];
 }
 
-// The per-pass fields of the status block (one thread).
-__device__ void clear_status(Status *st, bool keep_nonfinite)
+// The per-pass fields of the status block: the replica sums by the `nt`
+// threads of one workgroup, the rest by thread 0.
+__device__ void clear_status(Status *st, bool keep_nonfinite, int t, int nt)
 {
-    for (int r = 0; r < st->nrep && st->nrep > 1; ++r) {
-        st->rep_count[r] = 0;
-        st->rep_energy[r] = 0.0;
-    }
+    if (st->nrep > 1)
+        for (int r = t; r < st->nrep; r += nt) {
+            st->rep_count[r] = 0;
+            st->rep_energy[r] = 0.0;
+        }
+    if (t != 0) return;
     st->count[0] = 0;
     st->count[1] = 0;
     st->energy = 0.0;
@@ -152,18 +155,16 @@ __device__ void clear_status(Status *st, bool keep_nonfinite)
 
 __global__ void k_reset_status(Status *st, int keep_nonfinite)
 {
-    if (threadIdx.x == 0) {
-        st->halt = 0;
-        clear_status(st, keep_nonfinite != 0);
-    }
+    if (threadIdx.x == 0) st->halt = 0;
+    clear_status(st, keep_nonfinite != 0, threadIdx.x, blockDim.x);
 }
 
 // A run of replayed step graphs starts from a clear status block; each
 // replay's last kernel clears it again for the next (see k_update_fin).
 __global__ void k_begin_steps(Status *st, unsigned long long *counts)
 {
+    clear_status(st, false, threadIdx.x, blockDim.x);
     if (threadIdx.x == 0) {
-        clear_status(st, false);
         st->halt = 0;
         st->nstep = 0;
         st->counts = counts;
@@ -236,14 +237,17 @@ __global__ __launch_bounds__(NSLOT) void k_update_fin(DeviceState d, double hdt,
 {
     if (blockIdx.x == 0) {
         fold_slots(st);
-        if (record && threadIdx.x == 0 && !st->halt) {
-            if (st->nrep > 1) {
-                for (int r = 0; r < st->nrep; ++r) st->counts[(long long)st->nstep * st->nrep + r] = st->rep_count[r];
-            } else {
-                st->counts[st->nstep] = st->count[0];
+        __syncthreads();  // thread 0's fold (and halt) is visible to the workgroup
+        if (record && !st->halt) {
+            const int t = threadIdx.x, nrep = st->nrep, ns = st->nstep;
+            if (nrep > 1) {
+                for (int r = t; r < nrep; r += NSLOT) st->counts[(long long)ns * nrep + r] = st->rep_count[r];
+            } else if (t == 0) {
+                st->counts[ns] = st->count[0];
             }
-            st->nstep = st->nstep + 1;
-            clear_status(st, false);
+            __syncthreads();  // every read of this step's sums precedes the clear
+            clear_status(st, false, t, NSLOT);
+            if (t == 0) st->nstep = ns + 1;
         }
     }
     if (st->flags[2]) return;
