@@ -271,3 +271,53 @@ def test_oracle_cli_writes_reference_csv(oracle, tmp_path):
             assert len(cells) == 14
             for c in cells[1:]:
                 assert c == "%g" % float(c)
+
+
+def _at_gradient_longdouble(P):
+    """+grad E of the one triplet P (3 x 3), E = nu (1 + 3 cos cos cos)/(r r r)^3
+    with nu = 1, in 80-bit extended precision (numpy longdouble on x86-64):
+    the squared-distance form, whose only rounding is at 64-bit mantissas."""
+    ld = np.longdouble
+    xi, xj, xk = P.astype(ld)
+    dj, dk, djk = xj - xi, xk - xi, xk - xj
+    u, v, w = (dj * dj).sum(), (dk * dk).sum(), (djk * djk).sum()
+    K = ld(0.375)
+    s = 1 / np.sqrt(u * v * w)
+    s2 = s * s
+    A, B, C = u + v - w, u - v + w, -u + v + w
+    X, Y, Z = B * C, A * C, A * B
+    Q = A * X * s2
+    R = -ld(1.5) - ld(2.5) * K * Q
+    S = X + Y + Z
+    s5 = s2 * s2 * s
+    Eu = s5 * (R * v * w + K * (S - 2 * Z))
+    Ev = s5 * (R * u * w + K * (S - 2 * Y))
+    return -2 * Eu * dj - 2 * Ev * dk  # the force on the first particle
+
+
+def test_close_pairs_fast_form_more_accurate_than_reference(oracle):
+    """A triplet with a close pair (separation 3e-4 .. 1e-2) and a third
+    particle 0.3 .. 0.8 away: the third particle receives two huge, nearly
+    opposite terms.  Against an 80-bit evaluation, the squared-distance form
+    (the fast paths' formulation) stays within 5e-12 relative on that
+    particle; the reference expression (AxilrodTeller.cpp:22-93, its powers
+    and divisions of the side lengths) loses far more.  This is why, on systems
+    of millions of particles, the FP64 fast path and the reference differ by
+    ~1e-12 on a few particles: the difference is the reference's rounding."""
+    rng = np.random.default_rng(5)
+    L = np.array([50.0, 50.0, 50.0])
+    worst_ref = worst_fast = 0.0
+    for _ in range(40):
+        c = rng.uniform(2, 3, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        far = rng.normal(size=3)
+        far *= rng.uniform(0.3, 0.8) / np.linalg.norm(far)
+        P = np.array([c + far, c, c + 10 ** rng.uniform(-3.5, -2) * d])
+        truth = _at_gradient_longdouble(P).astype(float)
+        fr = oracle.sweep(P, cutoff=2.0, L=L, mode="owner")[-2][0]
+        ff = oracle.forces_fast(P, 1.0, 2.0, L)[0][0]
+        worst_ref = max(worst_ref, np.linalg.norm(fr - truth) / np.linalg.norm(truth))
+        worst_fast = max(worst_fast, np.linalg.norm(ff - truth) / np.linalg.norm(truth))
+    assert worst_fast <= 5e-12
+    assert worst_ref >= 10 * worst_fast
