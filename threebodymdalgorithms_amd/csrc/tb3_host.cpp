@@ -327,6 +327,7 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
         dev_alloc(g.cell_cnt, g.ncell);
         HIPCHK(hipMemset(g.cell_cnt, 0, (size_t)g.ncell * sizeof(int)));
         dev_alloc(g.cell_start, (long long)g.ncell + 1);
+        dev_alloc(g.order, g.ncell);
         c->grid_cell_cap = g.ncell;
     }
     if (n > c->grid_n_cap) {
@@ -762,6 +763,7 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.cell_of);
     dev_free(c->grid.cell_cnt);
     dev_free(c->grid.cell_start);
+    dev_free(c->grid.order);
     dev_free(c->grid.cell_idx);
     dev_free(c->scan_tmp);
     dev_free(c->triples);

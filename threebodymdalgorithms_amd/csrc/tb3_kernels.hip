@@ -476,13 +476,45 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
 // Places each particle in its cell's range (index and a cell-ordered copy of
 // its position), counting the cell's counter back down to zero so the
 // counters are clear for the next step.
-__global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
-                               const double *__restrict__ z, const int *__restrict__ cell_of,
-                               const int *__restrict__ cell_start, int *__restrict__ cell_cnt,
-                               int *__restrict__ cell_idx, double *__restrict__ xs, double *__restrict__ ys,
-                               double *__restrict__ zs)
+//
+// Workgroups 0..7 (or fewer, looping) also order the cells of each XCD's run
+// of (ncell + 7) / 8 consecutive cells by decreasing particle count into
+// `order` (run r at r * run + rank), for the cell kernel to start the
+// heaviest cells first: the cost of a cell grows roughly with the cube of its
+// count, and with only a few workgroups per slot the heaviest cells would
+// otherwise decide when the sweep ends.  A counting sort on the count (256
+// bins, counts clamped to 255; ties in any order).
+__global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__restrict__ x,
+                                                      const double *__restrict__ y, const double *__restrict__ z,
+                                                      const int *__restrict__ cell_of,
+                                                      const int *__restrict__ cell_start, int *__restrict__ cell_cnt,
+                                                      int *__restrict__ cell_idx, double *__restrict__ xs,
+                                                      double *__restrict__ ys, double *__restrict__ zs, int ncell,
+                                                      int *__restrict__ order)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int hist[256];
+    const int t = threadIdx.x;
+    const int run = (ncell + 7) >> 3;
+    for (int r = blockIdx.x; r < 8; r += gridDim.x) {
+        const int lo = r * run, hi = min(lo + run, ncell);
+        hist[t] = 0;
+        __syncthreads();
+        for (int c = lo + t; c < hi; c += 256) atomicAdd(&hist[255 - min(cell_start[c + 1] - cell_start[c], 255)], 1);
+        __syncthreads();
+        if (t == 0) {
+            int acc = 0;
+            for (int b = 0; b < 256; ++b) {
+                const int h = hist[b];
+                hist[b] = acc;
+                acc += h;
+            }
+        }
+        __syncthreads();
+        for (int c = lo + t; c < hi; c += 256)
+            order[lo + atomicAdd(&hist[255 - min(cell_start[c + 1] - cell_start[c], 255)], 1)] = c;
+        __syncthreads();
+    }
+    const int p = blockIdx.x * blockDim.x + t;
     if (p >= n) return;
     const int c = cell_of[p];
     const int slot = cell_start[c] + atomicSub(&cell_cnt[c], 1) - 1;
@@ -816,6 +848,7 @@ struct CellsDev {
     int ncell;
     int *owners;   // owners handed to the dense-owner kernel (st->dense_n of them)
     int dense_on;  // 0: the dense-owner kernel does not follow; an owner that needs it fails the pass
+    const int *order;  // each XCD's run of cells, heaviest first (k_cell_scatter)
 };
 
 // An owner that needs the dense-owner kernel when none follows this launch:
@@ -915,13 +948,15 @@ __device__ __forceinline__ int staged_slot(const int *cst, const int *cpre, int 
     return cst[k] + (t - cpre[k]);
 }
 
-// Cell for workgroup b: workgroups are dispatched round-robin over the 8
-// XCDs (b % 8), so each XCD gets a contiguous run of cells and the
-// neighbourhoods it stages overlap in its own L2.
-__device__ __forceinline__ int cell_for_block(int b, int ncell)
+// Cell for workgroup b, or -1: workgroups are dispatched round-robin over
+// the 8 XCDs (b % 8), so each XCD gets a contiguous run of cells, whose
+// neighbourhoods overlap in its own L2, and takes them heaviest first.
+__device__ __forceinline__ int cell_for_block(int b, int ncell, const int *order)
 {
-    const int chunk = (ncell + 7) >> 3;
-    return (b & 7) * chunk + (b >> 3);
+    const int run = (ncell + 7) >> 3;
+    const int r = b & 7, k = b >> 3;
+    if (k >= run || r * run + k >= ncell) return -1;
+    return order[r * run + k];
 }
 
 template <class T, bool GRAD, bool WANT_E>
@@ -941,8 +976,8 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     __shared__ int s_next, s_far, s_wcnt[CELL_WAVES];
 
     const CutDev &A = a.c;
-    const int c = cell_for_block(blockIdx.x, a.ncell);
-    if (c >= a.ncell) return;
+    const int c = cell_for_block(blockIdx.x, a.ncell, a.order);
+    if (c < 0) return;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid / WAVE;
     // A halted step-graph replay leaves the step to the host.  Non-finite
     // positions pair with everything: leave the sweep to the per-owner
@@ -2087,7 +2122,7 @@ void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double
     launch_scan_i32(s, g.cell_cnt, g.cell_start, g.ncell, tmp);
     if (n > 0)
         hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d.x, d.y, d.z, g.cell_of,
-                           g.cell_start, g.cell_cnt, g.cell_idx, g.xs, g.ys, g.zs);
+                           g.cell_start, g.cell_cnt, g.cell_idx, g.xs, g.ys, g.zs, g.ncell, g.order);
 }
 
 template <class K>
@@ -2138,6 +2173,7 @@ void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     a.zs = g.zs;
     a.ncell = g.ncell;
     a.owners = owners;
+    a.order = g.order;
     // one workgroup per cell, rounded up to a multiple of the 8 XCDs
     const dim3 gr((unsigned)(8 * ((g.ncell + 7) / 8))), blk(CELL_WAVES * WAVE);
 #define TB3_CELLS(T, G, E) hipLaunchKernelGGL((k_p3bca_cells<T, G, E>), gr, blk, 0, s, a)
