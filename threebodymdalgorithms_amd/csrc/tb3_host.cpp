@@ -297,6 +297,9 @@ void configure_grid(tb3_ctx *c)
         g.nrep = R;
     }
     publish_replicas(c);
+    // the per-cell work of the last sweep belongs to the old cell numbering
+    if (g.cell_work && g.ncell <= c->grid_cell_cap)
+        HIPCHK(hipMemset(g.cell_work, 0, (size_t)g.ncell * sizeof(unsigned)));
     c->cells_ok = true;  // a new grid gets a new chance at the cell kernel
     c->dense_on = false;
     ++c->epoch;
@@ -328,6 +331,8 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
         HIPCHK(hipMemset(g.cell_cnt, 0, (size_t)g.ncell * sizeof(int)));
         dev_alloc(g.cell_start, (long long)g.ncell + 1);
         dev_alloc(g.order, g.ncell);
+        dev_alloc(g.cell_work, g.ncell);
+        HIPCHK(hipMemset(g.cell_work, 0, (size_t)g.ncell * sizeof(unsigned)));
         c->grid_cell_cap = g.ncell;
     }
     if (n > c->grid_n_cap) {
@@ -764,6 +769,7 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.cell_cnt);
     dev_free(c->grid.cell_start);
     dev_free(c->grid.order);
+    dev_free(c->grid.cell_work);
     dev_free(c->grid.cell_idx);
     dev_free(c->scan_tmp);
     dev_free(c->triples);

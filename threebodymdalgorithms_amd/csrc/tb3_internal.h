@@ -30,6 +30,7 @@ struct CellGrid {
     int *cell_cnt = nullptr;    // ncell (zero between steps)
     int *cell_start = nullptr;  // ncell + 1
     int *order = nullptr;       // ncell: each XCD's run of (ncell + 7) / 8 cells, heaviest first
+    unsigned *cell_work = nullptr;  // ncell: accepted triplets of each cell's last sweep (0: none yet)
     int *cell_idx = nullptr;    // n, particle indices grouped by cell (order within a cell unspecified)
     double *xs = nullptr, *ys = nullptr, *zs = nullptr;  // n, positions in cell_idx order
 

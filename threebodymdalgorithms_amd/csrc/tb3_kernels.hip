@@ -490,16 +490,25 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
                                                       const int *__restrict__ cell_start, int *__restrict__ cell_cnt,
                                                       int *__restrict__ cell_idx, double *__restrict__ xs,
                                                       double *__restrict__ ys, double *__restrict__ zs, int ncell,
-                                                      int *__restrict__ order)
+                                                      int *__restrict__ order, const unsigned *__restrict__ cell_work)
 {
     __shared__ int hist[256];
     const int t = threadIdx.x;
     const int run = (ncell + 7) >> 3;
+    // A cell's work is the triplet count of its last sweep (most particles
+    // move little in a step), or 16 n^2 before it has one; bins of 1/16
+    // octave, heaviest first.
+    auto bin = [&](int c) {
+        const int n = cell_start[c + 1] - cell_start[c];
+        const unsigned w = cell_work[c];
+        const float k = w ? (float)w : 16.0f * (float)n * (float)n;
+        return 255 - min(255, (int)(16.0f * __log2f(1.0f + k)));
+    };
     for (int r = blockIdx.x; r < 8; r += gridDim.x) {
         const int lo = r * run, hi = min(lo + run, ncell);
         hist[t] = 0;
         __syncthreads();
-        for (int c = lo + t; c < hi; c += 256) atomicAdd(&hist[255 - min(cell_start[c + 1] - cell_start[c], 255)], 1);
+        for (int c = lo + t; c < hi; c += 256) atomicAdd(&hist[bin(c)], 1);
         __syncthreads();
         if (t == 0) {
             int acc = 0;
@@ -511,7 +520,7 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
         }
         __syncthreads();
         for (int c = lo + t; c < hi; c += 256)
-            order[lo + atomicAdd(&hist[255 - min(cell_start[c + 1] - cell_start[c], 255)], 1)] = c;
+            order[lo + atomicAdd(&hist[bin(c)], 1)] = c;
         __syncthreads();
     }
     const int p = blockIdx.x * blockDim.x + t;
@@ -736,12 +745,14 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
                 gpx = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gpx));
                 gpy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, gpy));
                 gpz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, gpz));
-                lds_add(&gx_[ia], __builtin_fma(cu, djx, -cw * ex));
-                lds_add(&gy_[ia], __builtin_fma(cu, djy, -cw * ey));
-                lds_add(&gz_[ia], __builtin_fma(cu, djz, -cw * ez));
-                lds_add(&gx_[ib], __builtin_fma(cv, dkx, cw * ex));
-                lds_add(&gy_[ib], __builtin_fma(cv, dky, cw * ey));
-                lds_add(&gz_[ib], __builtin_fma(cv, dkz, cw * ez));
+                // the j and k terms share the products cw * e: (-cw) e = -(cw e) exactly
+                const double tx = cw * ex, ty = cw * ey, tz = cw * ez;
+                lds_add(&gx_[ia], __builtin_fma(cu, djx, -tx));
+                lds_add(&gy_[ia], __builtin_fma(cu, djy, -ty));
+                lds_add(&gz_[ia], __builtin_fma(cu, djz, -tz));
+                lds_add(&gx_[ib], __builtin_fma(cv, dkx, tx));
+                lds_add(&gy_[ib], __builtin_fma(cv, dky, ty));
+                lds_add(&gz_[ib], __builtin_fma(cv, dkz, tz));
             }
             esum += (double)et;
             ++cnt;
@@ -849,6 +860,7 @@ struct CellsDev {
     int *owners;   // owners handed to the dense-owner kernel (st->dense_n of them)
     int dense_on;  // 0: the dense-owner kernel does not follow; an owner that needs it fails the pass
     const int *order;  // each XCD's run of cells, heaviest first (k_cell_scatter)
+    unsigned *cell_work;  // each cell's accepted triplets, the next sweep's dispatch key
 };
 
 // An owner that needs the dense-owner kernel when none follows this launch:
@@ -974,6 +986,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     __shared__ unsigned short ring[CELL_WAVES][RQ];
     __shared__ int cst[32], cpre[33], sinfo[4];
     __shared__ int s_next, s_far, s_wcnt[CELL_WAVES];
+    __shared__ unsigned s_work;
 
     const CutDev &A = a.c;
     const int c = cell_for_block(blockIdx.x, a.ncell, a.order);
@@ -1015,6 +1028,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     // outside the box) sends the workgroup to the periodic form.
     if (tid == 0) {
         s_next = 0;
+        s_work = 0;
         s_far = (sg.nc[0] < 7 || sg.nc[1] < 7 || sg.nc[2] < 7) ? 1 : 0;
     }
     int total = 0;
@@ -1073,7 +1087,9 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
         if (far) s_far = 1;
     }
     if (total > NBCAP) {
-        // hand every owner of c to the dense-owner kernel
+        // hand every owner of c to the dense-owner kernel (which makes c a
+        // light workgroup: it goes last in the next sweep's order)
+        if (tid == 0) a.cell_work[c] = 1;
         if (!a.dense_on) {
             if (tid == 0) want_dense_kernel(A.st);
             return;
@@ -1166,12 +1182,14 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
                         T et;
                         at_coeffs_from64<T, GRAD>(u64, v64, w64, nu, K, cu, cv, cw, et);
                         if (GRAD) {
-                            lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
-                            lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
-                            lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
-                            lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
-                            lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
-                            lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
+                            // the j and k terms share the products cw * e: (-cw) e = -(cw e) exactly
+                            const double tx = cw * ex, ty = cw * ey, tz = cw * ez;
+                            lds_add(&hx[r], __builtin_fma(cu, djx, -tx));
+                            lds_add(&hy[r], __builtin_fma(cu, djy, -ty));
+                            lds_add(&hz[r], __builtin_fma(cu, djz, -tz));
+                            lds_add(&hx[kk], __builtin_fma(cv, dkx, tx));
+                            lds_add(&hy[kk], __builtin_fma(cv, dky, ty));
+                            lds_add(&hz[kk], __builtin_fma(cv, dkz, tz));
                         }
                         if (WANT_E) esum += (double)et;
                         ++cnt;
@@ -1263,7 +1281,10 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
         if (WANT_E) add_energy(A.st, ee);
         add_count(A.st, A.count_slot, cw);
         add_rep(A, sg.r, cw, ee);
+        atomicAdd(&s_work, (unsigned)cw);
     }
+    __syncthreads();
+    if (tid == 0) a.cell_work[c] = max(s_work, 1u);
 }
 
 
@@ -1398,12 +1419,14 @@ __global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
                         gix = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gix));
                         giy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, giy));
                         giz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, giz));
-                        lds_add(&hx[r], __builtin_fma(cu, djx, -cw * ex));
-                        lds_add(&hy[r], __builtin_fma(cu, djy, -cw * ey));
-                        lds_add(&hz[r], __builtin_fma(cu, djz, -cw * ez));
-                        lds_add(&hx[kk], __builtin_fma(cv, dkx, cw * ex));
-                        lds_add(&hy[kk], __builtin_fma(cv, dky, cw * ey));
-                        lds_add(&hz[kk], __builtin_fma(cv, dkz, cw * ez));
+                        // the j and k terms share the products cw * e: (-cw) e = -(cw e) exactly
+                        const double tx = cw * ex, ty = cw * ey, tz = cw * ez;
+                        lds_add(&hx[r], __builtin_fma(cu, djx, -tx));
+                        lds_add(&hy[r], __builtin_fma(cu, djy, -ty));
+                        lds_add(&hz[r], __builtin_fma(cu, djz, -tz));
+                        lds_add(&hx[kk], __builtin_fma(cv, dkx, tx));
+                        lds_add(&hy[kk], __builtin_fma(cv, dky, ty));
+                        lds_add(&hz[kk], __builtin_fma(cv, dkz, tz));
                     }
                     esum += (double)et;
                     ++cnt;
@@ -2122,7 +2145,7 @@ void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double
     launch_scan_i32(s, g.cell_cnt, g.cell_start, g.ncell, tmp);
     if (n > 0)
         hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d.x, d.y, d.z, g.cell_of,
-                           g.cell_start, g.cell_cnt, g.cell_idx, g.xs, g.ys, g.zs, g.ncell, g.order);
+                           g.cell_start, g.cell_cnt, g.cell_idx, g.xs, g.ys, g.zs, g.ncell, g.order, g.cell_work);
 }
 
 template <class K>
@@ -2174,6 +2197,7 @@ void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     a.ncell = g.ncell;
     a.owners = owners;
     a.order = g.order;
+    a.cell_work = g.cell_work;
     // one workgroup per cell, rounded up to a multiple of the 8 XCDs
     const dim3 gr((unsigned)(8 * ((g.ncell + 7) / 8))), blk(CELL_WAVES * WAVE);
 #define TB3_CELLS(T, G, E) hipLaunchKernelGGL((k_p3bca_cells<T, G, E>), gr, blk, 0, s, a)
