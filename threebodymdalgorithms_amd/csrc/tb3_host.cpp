@@ -333,12 +333,15 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
         dev_alloc(g.order, g.ncell);
         dev_alloc(g.cell_work, g.ncell);
         HIPCHK(hipMemset(g.cell_work, 0, (size_t)g.ncell * sizeof(unsigned)));
+        dev_alloc(g.scatter_ticket, 1);
+        HIPCHK(hipMemset(g.scatter_ticket, 0, sizeof(unsigned)));
         c->grid_cell_cap = g.ncell;
     }
     if (n > c->grid_n_cap) {
         ++c->epoch;
         dev_alloc(c->d_owners, n);
         dev_alloc(g.cell_of, n);
+        dev_alloc(g.cell_rank, n);
         dev_alloc(g.cell_idx, n);
         dev_alloc(g.xs, n);
         dev_alloc(g.ys, n);
@@ -770,6 +773,8 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.cell_start);
     dev_free(c->grid.order);
     dev_free(c->grid.cell_work);
+    dev_free(c->grid.cell_rank);
+    dev_free(c->grid.scatter_ticket);
     dev_free(c->grid.cell_idx);
     dev_free(c->scan_tmp);
     dev_free(c->triples);

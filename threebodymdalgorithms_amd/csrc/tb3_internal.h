@@ -28,6 +28,8 @@ struct CellGrid {
     double L[3] = {1, 1, 1};
     int *cell_of = nullptr;     // n
     int *cell_cnt = nullptr;    // ncell (zero between steps)
+    int *cell_rank = nullptr;   // n: each particle's rank in its cell (grids of at most 8192 cells)
+    unsigned *scatter_ticket = nullptr;  // 1: finished scatter workgroups (returns to 0)
     int *cell_start = nullptr;  // ncell + 1
     int *order = nullptr;       // ncell: each XCD's run of (ncell + 7) / 8 cells, heaviest first
     unsigned *cell_work = nullptr;  // ncell: accepted triplets of each cell's last sweep (0: none yet)

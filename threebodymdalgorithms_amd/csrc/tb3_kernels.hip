@@ -452,8 +452,9 @@ __device__ __forceinline__ void cell_of_pos(const SysGrid &g, double x, double y
     }
 }
 
+template <bool RANK>
 __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_f, GridDev g,
-                      int *__restrict__ cell_of, int *__restrict__ cell_cnt, Status *st)
+                      int *__restrict__ cell_of, int *__restrict__ cell_cnt, int *__restrict__ cell_rank, Status *st)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.n) return;
@@ -470,36 +471,27 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
     cell_of_pos(sg, px, py, pz, cx, cy, cz);
     const int c = sg.cbase + (cz * sg.nc[1] + cy) * sg.nc[0] + cx;
     cell_of[p] = c;
-    atomicAdd(&cell_cnt[c], 1);
+    if constexpr (RANK) cell_rank[p] = atomicAdd(&cell_cnt[c], 1);
+    else atomicAdd(&cell_cnt[c], 1);
 }
 
-// Places each particle in its cell's range (index and a cell-ordered copy of
-// its position), counting the cell's counter back down to zero so the
-// counters are clear for the next step.
-//
-// Workgroups 0..7 (or fewer, looping) also order the cells of each XCD's run
-// of (ncell + 7) / 8 consecutive cells by decreasing particle count into
-// `order` (run r at r * run + rank), for the cell kernel to start the
-// heaviest cells first: the cost of a cell grows roughly with the cube of its
-// count, and with only a few workgroups per slot the heaviest cells would
-// otherwise decide when the sweep ends.  A counting sort on the count (256
-// bins, counts clamped to 255; ties in any order).
-__global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__restrict__ x,
-                                                      const double *__restrict__ y, const double *__restrict__ z,
-                                                      const int *__restrict__ cell_of,
-                                                      const int *__restrict__ cell_start, int *__restrict__ cell_cnt,
-                                                      int *__restrict__ cell_idx, double *__restrict__ xs,
-                                                      double *__restrict__ ys, double *__restrict__ zs, int ncell,
-                                                      int *__restrict__ order, const unsigned *__restrict__ cell_work)
+// Orders the cells of each XCD's run of (ncell + 7) / 8 consecutive cells
+// heaviest first into `order` (run r at r * run + rank), for the cell kernel
+// to start the longest cells first: with under two workgroups per slot, the
+// cells that start last otherwise decide when the sweep ends.  A cell's work
+// is the triplet count of its last sweep (most particles move little in a
+// step), or 16 n^2 for n particles before it has one.  A counting sort on
+// 1/16-octave bins of the work (256 bins; ties in any order), one workgroup
+// per run (workgroups 0..7, or fewer, looping).  start(c) is cell c's first
+// slot.
+template <class Start>
+__device__ void order_runs(int ncell, const Start &start, const unsigned *__restrict__ cell_work,
+                           int *__restrict__ order, int *hist)
 {
-    __shared__ int hist[256];
     const int t = threadIdx.x;
     const int run = (ncell + 7) >> 3;
-    // A cell's work is the triplet count of its last sweep (most particles
-    // move little in a step), or 16 n^2 before it has one; bins of 1/16
-    // octave, heaviest first.
     auto bin = [&](int c) {
-        const int n = cell_start[c + 1] - cell_start[c];
+        const int n = start(c + 1) - start(c);
         const unsigned w = cell_work[c];
         const float k = w ? (float)w : 16.0f * (float)n * (float)n;
         return 255 - min(255, (int)(16.0f * __log2f(1.0f + k)));
@@ -519,11 +511,25 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
             }
         }
         __syncthreads();
-        for (int c = lo + t; c < hi; c += 256)
-            order[lo + atomicAdd(&hist[bin(c)], 1)] = c;
+        for (int c = lo + t; c < hi; c += 256) order[lo + atomicAdd(&hist[bin(c)], 1)] = c;
         __syncthreads();
     }
-    const int p = blockIdx.x * blockDim.x + t;
+}
+
+// Places each particle in its cell's range (index and a cell-ordered copy of
+// its position), counting the cell's counter back down to zero so the
+// counters are clear for the next step; also orders the cells (order_runs).
+__global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__restrict__ x,
+                                                      const double *__restrict__ y, const double *__restrict__ z,
+                                                      const int *__restrict__ cell_of,
+                                                      const int *__restrict__ cell_start, int *__restrict__ cell_cnt,
+                                                      int *__restrict__ cell_idx, double *__restrict__ xs,
+                                                      double *__restrict__ ys, double *__restrict__ zs, int ncell,
+                                                      int *__restrict__ order, const unsigned *__restrict__ cell_work)
+{
+    __shared__ int hist[256];
+    order_runs(ncell, [&](int c) { return cell_start[c]; }, cell_work, order, hist);
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int c = cell_of[p];
     const int slot = cell_start[c] + atomicSub(&cell_cnt[c], 1) - 1;
@@ -531,6 +537,71 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
     xs[slot] = x[p];
     ys[slot] = y[p];
     zs[slot] = z[p];
+}
+
+// Grids of at most FUSED_SCAN_CELLS cells: the scan of the counts is done by
+// every workgroup of the scatter in its own LDS, which saves the scan kernel
+// and a kernel boundary per step.  The binning pass recorded each particle's
+// rank in its cell, so the slot is start + rank and the counters are only
+// read here; the last workgroup to finish (a ticket that wraps back to 0)
+// clears them for the next step, after every workgroup has read them.
+// Workgroup 0 publishes the starts for the sweep kernels.
+constexpr int FUSED_SCAN_CELLS = 8192;
+
+__global__ __launch_bounds__(256) void k_cell_scatter_fused(
+    int n, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const int *__restrict__ cell_of, const int *__restrict__ cell_rank, int *__restrict__ cell_cnt,
+    int *__restrict__ cell_start, int *__restrict__ cell_idx, double *__restrict__ xs, double *__restrict__ ys,
+    double *__restrict__ zs, int ncell, int *__restrict__ order, const unsigned *__restrict__ cell_work,
+    unsigned *__restrict__ ticket)
+{
+    __shared__ int lst[FUSED_SCAN_CELLS + 1];
+    __shared__ int part[256];
+    __shared__ int hist[256];
+    __shared__ int s_last;
+    const int t = threadIdx.x;
+    // exclusive scan of the counts: coalesced load, a serial scan of each
+    // thread's chunk, then a scan of the 256 chunk sums
+    for (int c = t; c < ncell; c += 256) lst[c] = cell_cnt[c];
+    __syncthreads();
+    const int per = (ncell + 255) >> 8;
+    const int c0 = min(t * per, ncell), c1 = min(c0 + per, ncell);
+    int sum = 0;
+    for (int c = c0; c < c1; ++c) {
+        const int k = lst[c];
+        lst[c] = sum;
+        sum += k;
+    }
+    part[t] = sum;
+    int v = sum;
+    for (int o = 1; o < 256; o <<= 1) {
+        __syncthreads();
+        const int u = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        v += u;
+        part[t] = v;
+    }
+    const int off = v - sum;
+    for (int c = c0; c < c1; ++c) lst[c] += off;
+    if (t == 255) lst[ncell] = v;
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int c = t; c <= ncell; c += 256) cell_start[c] = lst[c];
+    order_runs(ncell, [&](int c) { return lst[c]; }, cell_work, order, hist);
+
+    const int p = blockIdx.x * blockDim.x + t;
+    if (p < n) {
+        const int slot = lst[cell_of[p]] + cell_rank[p];
+        cell_idx[slot] = p;
+        xs[slot] = x[p];
+        ys[slot] = y[p];
+        zs[slot] = z[p];
+    }
+    __syncthreads();
+    if (t == 0) s_last = atomicInc(ticket, gridDim.x - 1) == gridDim.x - 1;
+    __syncthreads();
+    if (s_last)
+        for (int c = t; c < ncell; c += 256) cell_cnt[c] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2139,9 +2210,17 @@ void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double
                         CellGrid &g, Status *st, long long *tmp)
 {
     const int n = d.n;
+    if (n > 0 && g.ncell <= FUSED_SCAN_CELLS) {
+        hipLaunchKernelGGL(k_bin<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, predict, dt, h, zero_f,
+                           grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank, st);
+        hipLaunchKernelGGL(k_cell_scatter_fused, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d.x, d.y, d.z,
+                           g.cell_of, g.cell_rank, g.cell_cnt, g.cell_start, g.cell_idx, g.xs, g.ys, g.zs, g.ncell,
+                           g.order, g.cell_work, g.scatter_ticket);
+        return;
+    }
     if (n > 0)
-        hipLaunchKernelGGL(k_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, predict, dt, h, zero_f, grid_dev(g),
-                           g.cell_of, g.cell_cnt, st);
+        hipLaunchKernelGGL(k_bin<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, predict, dt, h, zero_f,
+                           grid_dev(g), g.cell_of, g.cell_cnt, nullptr, st);
     launch_scan_i32(s, g.cell_cnt, g.cell_start, g.ncell, tmp);
     if (n > 0)
         hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d.x, d.y, d.z, g.cell_of,
