@@ -246,8 +246,18 @@ def main():
         if not args.skip_variants:
             ex = timed_steps(eng, P, "exact", args.steps, args.warmup, sync)
             log(f"bench: exact {ex['ms_per_step']:.3f} ms/step force kernel {ex['force_kernel_ms']:.3f} ms")
-            if ex["counts"] != fp64["counts"]:
-                log(f"bench: WARNING exact and fp64 counts differ: {ex['counts']} vs {fp64['counts']}")
+            # step 0 starts from identical positions and must count the same;
+            # later steps follow trajectories that the fast path's ~1e-13
+            # force differences move apart until a triplet crosses the cutoff
+            # differently
+            same = next((k for k, (a, b) in enumerate(zip(ex["counts"], fp64["counts"])) if a != b),
+                        len(ex["counts"]))
+            if same == 0:
+                log(f"bench: WARNING exact and fp64 step-0 counts differ: {ex['counts']} vs {fp64['counts']}")
+            elif same < len(ex["counts"]):
+                worst = max(abs(a - b) for a, b in zip(ex["counts"], fp64["counts"]))
+                log(f"bench: exact and fp64 counts agree for steps 0-{same - 1}; the trajectories then diverge "
+                    f"(at most {worst} triplets apart)")
             f32 = timed_steps(eng, P, "fp32", args.steps, args.warmup, sync, not args.no_autotune)
             acc = fp32_accuracy(eng, P)
             log(f"bench: fp32 {f32['ms_per_step']:.3f} ms/step force kernel {f32['force_kernel_ms']:.3f} ms {acc}")

@@ -126,6 +126,38 @@ def test_exact_trajectory_bit_identical(eng, oracle):
         assert np.array_equal(a, b), name
 
 
+def test_large_grid_three_kernel_cell_list(eng, oracle):
+    """Grids of more than 8192 cells build the cell list with a separate
+    count scan (smaller grids scan inside the scatter kernel): cutoff 0.7 in
+    a 21.5^3 box gives 30^3 cells.  Three exact steps bit for bit (the cell
+    counters must come back clear after every step), then one fast FP64
+    sweep with exact counts."""
+    n = 20000
+    box = (n / 2.0) ** (1 / 3)
+    P = uniform_particles(n, box, 91)
+    V, A, M = np.zeros_like(P), np.zeros_like(P), np.ones(n)
+    g = np.array([0.0, -9.81, 0.0])
+    eng.set_particles(P, V, A, M)
+    eng.configure("p3bca", cutoff=0.7, precision="exact")
+    say("exact trajectory p3bca n=20000 cutoff 0.7 (27000 cells) x3")
+    counts = [eng.step(0.001, g)[1] for _ in range(3)]
+    st = eng.get_particles()
+    pos, vel, acc, f, oc = oracle.simulate(P, V, A, M, "p3bca", 3, 0.001, g, cutoff=0.7)
+    assert counts == [int(c) for c in oc]
+    for name, a, b in (("pos", st["pos"], pos), ("vel", st["vel"], vel), ("acc", st["acc"], acc),
+                       ("f", st["f"], f)):
+        assert np.array_equal(a, b), name
+    L, _ = oracle.domain_size(P)
+    say("fp64 p3bca n=20000 cutoff 0.7")
+    f, cnt = gpu_forces(eng, P, "p3bca", 0.7, "fp64")
+    *_, fo, co = oracle.sweep(P, cutoff=0.7, L=L, mode="owner")
+    assert cnt == co
+    e = force_errors(f, fo)
+    say(f"  fp64 errors {e}")
+    assert e["max_rel_floored"] <= 1e-12 and e["rms_rel"] <= 1e-12
+    assert e["max_rel"] <= 1e-10
+
+
 # 12.6: 6 cells per dimension (the cutoff tests keep the periodic form);
 # 16.0: 7 cells, where the plain-difference form serves the inner cells
 @pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0),

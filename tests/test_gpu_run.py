@@ -187,3 +187,25 @@ def test_autotune_keeps_results(eng):
     den = np.linalg.norm(f0, axis=1)
     num = np.linalg.norm(f1 - f0, axis=1)
     assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max() <= 1e-12
+
+
+def test_run_after_new_particles(eng):
+    """A new particle set in an unchanged grid (fixed domain, same count)
+    reuses the captured step graph; another particle count must capture
+    again.  Each run equals a fresh engine's one-step-at-a-time run, bit for
+    bit on the exact path."""
+    L = np.array([12.0, 12.0, 12.0])
+    sets = [uniform_particles(3000, 12.0, 11), uniform_particles(3000, 12.0, 12),
+            uniform_particles(2500, 12.0, 13), uniform_particles(3000, 12.0, 12)]
+    eng.set_particles(sets[0], np.zeros_like(sets[0]), np.zeros_like(sets[0]), np.ones(len(sets[0])))
+    eng.configure("p3bca", cutoff=2.0, precision="exact", domain_L=L)
+    eng.run(4, 0.001, G)
+    for k, P in enumerate(sets[1:]):
+        say(f"run after set_particles #{k + 1} n={len(P)}")
+        Z = np.zeros_like(P)
+        eng.set_particles(P, Z, Z, np.ones(len(P)))
+        counts = [int(c) for c in eng.run(4, 0.001, G)]
+        st = eng.get_particles()
+        with tb3.Engine(0) as fresh:
+            ref, rc = state_after(fresh, P, Z, "p3bca", 2.0, "exact", [("step", 1)] * 4, L=L)
+        assert_same(st, counts, ref, rc, exact=True)

@@ -255,6 +255,14 @@ void publish_replicas(tb3_ctx *c)
 void configure_grid(tb3_ctx *c)
 {
     CellGrid &g = c->grid;
+    const int old_nrep = g.nrep, old_ncell = g.ncell;
+    int old_nc[3];
+    double old_lo[3], old_L[3];
+    for (int k = 0; k < 3; ++k) {
+        old_nc[k] = g.nc[k];
+        old_lo[k] = g.lo[k];
+        old_L[k] = g.L[k];
+    }
     if (c->nrep <= 1) {
         grid_shape(c, c->L, g.nc);
         g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
@@ -297,12 +305,17 @@ void configure_grid(tb3_ctx *c)
         g.nrep = R;
     }
     publish_replicas(c);
+    // A single system whose grid comes out identical (the same particle set
+    // again, say) keeps its step graph and its per-cell work.
+    bool same = c->nrep <= 1 && old_nrep == 1 && g.ncell == old_ncell;
+    for (int k = 0; k < 3; ++k)
+        same = same && g.nc[k] == old_nc[k] && g.lo[k] == old_lo[k] && g.L[k] == old_L[k];
     // the per-cell work of the last sweep belongs to the old cell numbering
-    if (g.cell_work && g.ncell <= c->grid_cell_cap)
+    if (!same && g.cell_work && g.ncell <= c->grid_cell_cap)
         HIPCHK(hipMemset(g.cell_work, 0, (size_t)g.ncell * sizeof(unsigned)));
     c->cells_ok = true;  // a new grid gets a new chance at the cell kernel
     c->dense_on = false;
-    ++c->epoch;
+    if (!same) ++c->epoch;
 }
 
 // Per-replica domains: the bbox rule on each replica's particles, or the
@@ -841,11 +854,15 @@ tb3_status tb3_set_particles(tb3_ctx *c, int64_t n, const int32_t *id, const dou
     if (n < 0 || n > (int64_t)INT32_MAX / 16) return fail(c, TB3_EINVAL, "particle count out of range");
     if (n > 0 && (!pos || !vel || !acc || !mass)) return fail(c, TB3_EINVAL, "pos, vel, acc and mass are required");
     return guarded(c, [&] {
+        // the step graph bakes in n and the state arrays
+        if (n != c->n || c->nrep != 1) ++c->epoch;
         c->n = n;
         c->nrep = 1;  // a new particle set is one system until tb3_batch_set splits it
         c->ids.resize((size_t)n);
         for (int64_t i = 0; i < n; ++i) c->ids[(size_t)i] = id ? id[i] : (int32_t)i;
+        const double *old_block = c->state.block;
         c->state.ensure(n);
+        if (c->state.block != old_block) ++c->epoch;
         DeviceState &d = c->state.d;
         if (n > 0) {
             // stage n x 3 rows through the role accumulators, then split into SoA
