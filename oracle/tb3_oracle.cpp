@@ -8,6 +8,7 @@
 #include "tb3_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -233,6 +234,77 @@ inline bool pair_ok(const double *pos, int a, int b, double cutoff, double c2, c
     if (!(cutoff > 0)) return true;
     return !(sqr_dist_periodic(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], pos[3 * b], pos[3 * b + 1],
                                pos[3 * b + 2], L) > c2);
+}
+
+/* Algorithm::calculateInteractions with the owner rules of a multi-rank run
+ * (Algorithm.cpp:51-140): j starts at i+1 when b1 and b0 have one owner,
+ * else at 0 (77); k starts at j+1 when b2 and b1 have one owner, else at i+1
+ * when b2 and b0 do, else at 0 (81-87); b0 rows [start, start + steps), or
+ * every row when steps is -1 (72).  No cutoff: NATA and AUTA call the form
+ * without one (Algorithm.cpp:23-29). */
+uint64_t sweep_aos_owned(std::vector<Particle> &b0, std::vector<Particle> &b1, std::vector<Particle> &b2, int o0,
+                         int o1, int o2, int64_t start, int64_t steps, PotentialBase *pot)
+{
+    uint64_t act = 0;
+    const size_t iend = steps != -1 ? (size_t)(start + steps) : b0.size();
+    for (size_t i = (size_t)start; i < iend; ++i) {
+        if (b0[i].isDummy) continue;
+        const size_t jstart = o1 == o0 ? i + 1 : 0;
+        for (size_t j = jstart; j < b1.size(); ++j) {
+            if (b1[j].isDummy) continue;
+            size_t kstart = 0;
+            if (o2 == o1)
+                kstart = j + 1;
+            else if (o2 == o0)
+                kstart = i + 1;
+            for (size_t k = kstart; k < b2.size(); ++k) {
+                if (b2[k].isDummy) continue;
+                pot->CalculateForces(b0[i], b1[j], b2[k]);
+                act++;
+            }
+        }
+    }
+    return act;
+}
+
+/* AtomDecomposition::Init (AtomDecomposition.cpp:6-40): the first
+ * p - (n mod p) ranks hold floor(n/p) consecutive particles plus one dummy
+ * (when p does not divide n), the others floor(n/p) + 1, so every buffer has
+ * the same length. */
+std::vector<Particle> atom_share(int64_t n, const double *pos, int p, int r, int64_t *offset, int64_t *count)
+{
+    const int64_t normal = n / p, procs_normal = p - (int)(n % p);
+    int64_t cnt, off;
+    if (r < procs_normal) {
+        cnt = normal;
+        off = r * normal;
+    } else {
+        cnt = normal + 1;
+        off = procs_normal * normal + (r - procs_normal) * cnt;
+    }
+    std::vector<Particle> b = make_buffer(cnt, pos + 3 * off);
+    for (int64_t t = 0; t < cnt; ++t) b[(size_t)t].ID = (int)(off + t);
+    if (r < procs_normal && n % p != 0) {
+        Particle d;
+        std::memset(&d, 0, sizeof(d));
+        d.isDummy = true;
+        b.push_back(d);
+    }
+    *offset = off;
+    *count = cnt;
+    return b;
+}
+
+inline int ring_mod(int a, int p) { return ((a % p) + p) % p; }
+
+/* Utility::Triplet::operator== (utility/structs.hpp:164-169): equal as
+ * unordered triples. */
+inline bool same_triplet(const int *a, int b0, int b1, int b2)
+{
+    int x[3] = {a[0], a[1], a[2]}, y[3] = {b0, b1, b2};
+    std::sort(x, x + 3);
+    std::sort(y, y + 3);
+    return x[0] == y[0] && x[1] == y[1] && x[2] == y[2];
 }
 
 }  // namespace
@@ -652,6 +724,139 @@ double orc_sweep_cells_fast(int64_t n, const double *pos, double nu, double cuto
     }
     if (count) *count = cnt;
     return E;
+}
+
+int orc_sweep_ranks(int algo, int ranks, int64_t n, const double *pos, double nu, double *f,
+                    uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed, int64_t cap,
+                    int64_t *nprocessed)
+{
+    if ((algo != 0 && algo != 2) || ranks < 1 || n < 0) return -1;
+    const int p = ranks;
+    AxilrodTellerOracle at(nu);
+    g_potential_slot = &at;
+    /* copies[k][q]: rank q's particles in role buffer k (b0, b1, b2), each a
+     * separate accumulator that travels with its buffer (SimulationStep:
+     * b1 = b0, b2 = b0, NATA.cpp:87-99, AUTA.cpp:236-256) */
+    std::vector<std::vector<Particle>> copies[3];
+    std::vector<int64_t> off((size_t)p), cnt((size_t)p);
+    for (int k = 0; k < 3; ++k) copies[k].resize((size_t)p);
+    for (int q = 0; q < p; ++q) {
+        copies[0][(size_t)q] = atom_share(n, pos, p, q, &off[(size_t)q], &cnt[(size_t)q]);
+        copies[1][(size_t)q] = copies[0][(size_t)q];
+        copies[2][(size_t)q] = copies[0][(size_t)q];
+    }
+    /* hold[k][r]: whose role-k buffer rank r holds; a shift to the right
+     * neighbour (MPI_Sendrecv_replace, NATA.cpp:62-63, AUTA.cpp:30-31) hands
+     * rank r the buffer rank r-1 held */
+    std::vector<int> hold[3];
+    for (int k = 0; k < 3; ++k) {
+        hold[k].resize((size_t)p);
+        for (int r = 0; r < p; ++r) hold[k][(size_t)r] = r;
+    }
+    auto shift = [&](int k) {
+        std::vector<int> h = hold[k];
+        for (int r = 0; r < p; ++r) hold[k][(size_t)r] = h[(size_t)ring_mod(r - 1, p)];
+    };
+    for (int r = 0; r < p; ++r) buffer_counts[r] = particle_counts[r] = 0;
+    int64_t np = 0;
+    auto record = [&](int r, int a, int b, int c) {
+        if (processed && np < cap) {
+            processed[4 * np] = r;
+            processed[4 * np + 1] = a;
+            processed[4 * np + 2] = b;
+            processed[4 * np + 3] = c;
+        }
+        ++np;
+    };
+    if (algo == 0) {
+        /* NATA::SimulationStep (NATA.cpp:84-196): b0 stays; b2 shifts after
+         * every inner step, b1 after every outer one; a buffer triplet is
+         * computed by the first rank (in rank order) whose triplet is new
+         * among those of all ranks at that step (calculateProcessed, 27-52) */
+        std::vector<std::array<int, 3>> done;
+        int step = 0;
+        for (int i = 0; i < p; ++i) {
+            for (int j = 0; j < p; ++j) {
+                std::vector<char> calc((size_t)p, 0);
+                for (int x = 0; x < p; ++x) {
+                    const int b1 = ring_mod(x - step / p, p), b2 = ring_mod(x - step, p);
+                    bool seen = false;
+                    for (const auto &t : done) seen = seen || same_triplet(t.data(), x, b1, b2);
+                    if (!seen) {
+                        done.push_back({x, b1, b2});
+                        calc[(size_t)x] = 1;
+                    }
+                }
+                for (int r = 0; r < p; ++r) {
+                    if (!calc[(size_t)r]) continue;
+                    const int o1 = ring_mod(r - step / p, p), o2 = ring_mod(r - step, p);
+                    particle_counts[r] +=
+                        sweep_aos_owned(copies[0][(size_t)r], copies[1][(size_t)hold[1][(size_t)r]],
+                                        copies[2][(size_t)hold[2][(size_t)r]], r, o1, o2, 0, -1, g_potential_slot);
+                    buffer_counts[r]++;
+                    record(r, r, o1, o2);
+                }
+                if (p > 1) {
+                    shift(2);
+                    ++step;
+                }
+            }
+            if (p > 1) shift(1);
+        }
+    } else {
+        /* AUTA::SimulationStep (AUTA.cpp:236-383): all three buffers travel;
+         * rounds s = p, p-3, ... shift buffer i (2, 0, 1, ...) before every
+         * step but the very first, and every rank computes every step; with
+         * 3 | p one more shift, and each rank computes a third of the rows
+         * of the buffers sorted by owner (calculateOneThirdOfInteractions,
+         * 48-119); the buffers then go home (sendBackParticles) */
+        int i = 2;
+        for (int s = p; s > 0; s -= 3) {
+            for (int j = 0; j < s; ++j) {
+                if (j != 0 || s != p) shift(i);
+                for (int r = 0; r < p; ++r) {
+                    const int o0 = hold[0][(size_t)r], o1 = hold[1][(size_t)r], o2 = hold[2][(size_t)r];
+                    particle_counts[r] += sweep_aos_owned(copies[0][(size_t)o0], copies[1][(size_t)o1],
+                                                          copies[2][(size_t)o2], o0, o1, o2, 0, -1,
+                                                          g_potential_slot);
+                    buffer_counts[r]++;
+                    record(r, o0, o1, o2);
+                }
+            }
+            i = (i + 1) % 3;
+        }
+        if (p % 3 == 0) {
+            shift(i);
+            for (int r = 0; r < p; ++r) {
+                int o[3] = {hold[0][(size_t)r], hold[1][(size_t)r], hold[2][(size_t)r]};
+                if (o[0] == o[1] || o[0] == o[2] || o[1] == o[2]) return -1; /* the reference sorts distinct owners only */
+                int slot[3] = {0, 1, 2};
+                std::sort(slot, slot + 3, [&](int a, int b) { return o[a] < o[b]; });
+                std::vector<Particle> &s0 = copies[slot[0]][(size_t)o[slot[0]]];
+                std::vector<Particle> &s1 = copies[slot[1]][(size_t)o[slot[1]]];
+                std::vector<Particle> &s2 = copies[slot[2]][(size_t)o[slot[2]]];
+                const int third = r / (p / 3);
+                const int64_t sz = (int64_t)s0.size(), start = third * (sz / 3);
+                const int64_t steps = third == 2 ? sz - 2 * (sz / 3) : sz / 3;
+                particle_counts[r] += sweep_aos_owned(s0, s1, s2, o[slot[0]], o[slot[1]], o[slot[2]], start, steps,
+                                                      g_potential_slot);
+                buffer_counts[r]++;
+                record(r, o[0], o[1], o[2]);
+            }
+        }
+    }
+    /* SumUpParticles on each rank's own buffers (Algorithm.cpp:245-249) */
+    for (int q = 0; q < p; ++q)
+        for (int64_t t = 0; t < cnt[(size_t)q]; ++t) {
+            const Particle &a = copies[0][(size_t)q][(size_t)t], &b = copies[1][(size_t)q][(size_t)t],
+                           &c = copies[2][(size_t)q][(size_t)t];
+            double *fp = f + 3 * (off[(size_t)q] + t);
+            fp[0] = a.fX + (b.fX + c.fX);
+            fp[1] = a.fY + (b.fY + c.fY);
+            fp[2] = a.fZ + (b.fZ + c.fZ);
+        }
+    if (nprocessed) *nprocessed = np;
+    return 0;
 }
 
 }  // extern "C"

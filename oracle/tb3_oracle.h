@@ -117,6 +117,21 @@ double orc_sweep_cells_fast(int64_t n, const double *pos, double nu, double cuto
 uint64_t orc_sweep_brute_rows(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f0,
                               double *f1, double *f2, int64_t rows, uint64_t *accepted);
 
+/* A multi-rank step sweep (mpiexec -n ranks) restated in one process, for
+ * NATA (algo 0) and AUTA (algo 2), which run without a cutoff: the
+ * AtomDecomposition split with dummy padding, each algorithm's ring schedule
+ * of buffer triplets, the owner rules of the triplet loops and the role sum
+ * on each rank.  f receives the summed force of each particle (particle
+ * order); buffer_counts[r] and particle_counts[r] the two counts rank r
+ * returns from SimulationStep; processed, if non-NULL, up to cap rows of
+ * (rank, owner of b0, of b1, of b2) in the order computed, and *nprocessed
+ * their total.  Returns 0, or -1 for another algorithm or ranks < 1.  P3BCA
+ * at more than one rank (its Cartesian domain decomposition) is not
+ * restated. */
+int orc_sweep_ranks(int algo, int ranks, int64_t n, const double *pos, double nu, double *f,
+                    uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed, int64_t cap,
+                    int64_t *nprocessed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,10 @@ def lib():
         L.orc_sweep_brute_rows.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
                                            _dp, ctypes.c_int64, _u64p]
         L.orc_sweep_brute_rows.restype = ctypes.c_uint64
+        L.orc_sweep_ranks.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_double, _dp, _u64p,
+                                      _u64p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_int64)]
+        L.orc_sweep_ranks.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -98,6 +102,27 @@ def sweep(pos, nu=1.0, cutoff=-1.0, L=None, mode="owner", threads=0):
         cnt = lib().orc_sweep_owner(n, _p(pos), nu, cutoff, Lp, _p(f0), _p(f1), _p(f2), threads)
     lib().orc_sum_roles(n, _p(f0), _p(f1), _p(f2), _p(f))
     return f0, f1, f2, f, int(cnt)
+
+
+def sweep_ranks(pos, algo, ranks, nu=1.0):
+    """The step sweep of NATA or AUTA as `mpiexec -n ranks` runs it, restated
+    in one process: returns (f, buffer_counts, particle_counts, processed)
+    with processed a (k, 4) array of (rank, owner of b0, b1, b2) rows."""
+    pos = _c(pos)
+    n = pos.shape[0]
+    f = np.zeros((n, 3))
+    bc = np.zeros(ranks, dtype=np.uint64)
+    pc = np.zeros(ranks, dtype=np.uint64)
+    cap = 4 * (ranks + 2) ** 3
+    proc = np.zeros((cap, 4), dtype=np.int32)
+    nproc = ctypes.c_int64(0)
+    rc = lib().orc_sweep_ranks(ALGO[algo], ranks, n, _p(pos), nu, _p(f), bc.ctypes.data_as(_u64p),
+                               pc.ctypes.data_as(_u64p), proc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap,
+                               ctypes.byref(nproc))
+    if rc != 0:
+        raise ValueError(f"orc_sweep_ranks({algo}, {ranks}) returned {rc}")
+    assert nproc.value <= cap
+    return f, bc, pc, proc[:nproc.value]
 
 
 def brute_possible(pos, cutoff=-1.0, L=None, nu=1.0):

@@ -321,3 +321,88 @@ def test_close_pairs_fast_form_more_accurate_than_reference(oracle):
         worst_fast = max(worst_fast, np.linalg.norm(ff - truth) / np.linalg.norm(truth))
     assert worst_fast <= 5e-12
     assert worst_ref >= 10 * worst_fast
+
+
+def _unique_rank_triplets(p):
+    """generateAllUniqueTriplets (tests.cpp:337-348): i <= j <= k < p."""
+    return sorted((i, j, k) for i in range(p) for j in range(i, p) for k in range(j, p))
+
+
+@pytest.mark.parametrize("algo", ["nata", "auta"])
+@pytest.mark.parametrize("ranks", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+def test_multi_rank_sweep(oracle, algo, ranks):
+    """The step sweep as `mpiexec -n ranks` runs it (NATA, AUTA), restated in
+    one process.  Pinned by:
+      * the reference's coverage tests (tests.cpp:570-690): over all ranks,
+        the computed buffer triplets are every unique triplet of ranks once.
+        AUTA with 3 | p computes its last buffer triplets in thirds of their
+        rows, one third per group of p/3 ranks, so each of those appears
+        three times, once per third;
+      * every particle triplet once: the counts sum to C(N,3), with N = 61
+        not a multiple of p (dummy-padded buffers);
+      * the forces equal the one-process sweep up to reordering (the sums run
+        in another order), and bit for bit at p = 1.
+    The loop rule for a b2 that shares only b0's owner (Algorithm.cpp:84-86)
+    never arises in these two schedules (NATA finds such triplets already
+    processed; AUTA's rotation never forms them), so nothing here pins it."""
+    from math import comb
+    n = 61
+    P = uniform_particles(n, 5.0, 30 + ranks)
+    f, bc, pc, proc = oracle.sweep_ranks(P, algo, ranks)
+    *_, fo, co = oracle.sweep(P, mode="brute")
+    assert int(pc.sum()) == comb(n, 3) == co
+    assert len(proc) == int(bc.sum())
+    rows = [tuple(int(x) for x in r) for r in proc]
+    trip = [tuple(sorted(r[1:])) for r in rows]
+    if algo == "auta" and ranks % 3 == 0:
+        last = rows[-ranks:]
+        thirds = {}
+        for r, *_ in last:
+            thirds.setdefault(tuple(sorted(_)), set()).add(r // (ranks // 3))
+        assert all(t == {0, 1, 2} for t in thirds.values())
+        assert sorted(trip[:-ranks] + list(thirds)) == _unique_rank_triplets(ranks)
+    else:
+        assert sorted(trip) == _unique_rank_triplets(ranks)
+    if ranks == 1:
+        assert np.array_equal(f, fo) and list(bc) == [1]
+    else:
+        num = np.linalg.norm(f - fo, axis=1)
+        den = np.linalg.norm(fo, axis=1)
+        assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max() <= 1e-12
+
+
+def test_multi_rank_unsupported(oracle):
+    with pytest.raises(ValueError):
+        oracle.sweep_ranks(uniform_particles(10, 3.0, 1), "p3bca", 2)
+
+
+def test_oracle_cli_ranks_matches_one_process(oracle, tmp_path):
+    """`--ranks 4` runs the same trajectory as one process up to the
+    reordered sums, and prints each rank's two counts, whose particle counts
+    add up to C(N,3) per step."""
+    from math import comb
+    n = 30
+    P = uniform_particles(n, 3.0, 8)
+    src = tmp_path / "in.csv"
+    with open(src, "w") as fh:
+        fh.write("ID, pX, pY, pZ, vX, vY, vZ, aX, aY, aZ, m\n")
+        for i in range(n):
+            fh.write(f"{i}, {P[i,0]:.17g}, {P[i,1]:.17g}, {P[i,2]:.17g}, 0, 0, 0, 0, 0, 0, 1\n")
+    outs = {}
+    for ranks in (1, 4):
+        out = tmp_path / f"r{ranks}.csv"
+        r = subprocess.run([oracle.ORACLE_CLI, "-a", "nata", "-i", "2", "-d", "0.001", "-gy", "-9.81", "-csv",
+                            str(src), "-o", str(out), "--full-precision", "--ranks", str(ranks), "--counts"],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        outs[ranks] = np.loadtxt(tmp_path / f"r{ranks}_1.csv", delimiter=",", skiprows=1)
+        if ranks == 4:
+            per_step = {}
+            for line in r.stdout.splitlines():
+                if line.startswith("step ") and "proc" in line:
+                    s = int(line.split()[1].rstrip(":"))
+                    per_step[s] = per_step.get(s, 0) + int(line.split(": ")[2].split()[0])
+            assert per_step == {0: comb(n, 3), 1: comb(n, 3)}
+    a, b = outs[4], outs[1]
+    assert np.array_equal(a[:, 0], b[:, 0])
+    assert np.allclose(a[:, 1:], b[:, 1:], rtol=1e-11, atol=1e-11)
