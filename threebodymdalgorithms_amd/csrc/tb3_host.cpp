@@ -148,6 +148,7 @@ struct tb3_ctx {
     bool graph_cells_ok = false;
     bool graph_dense_on = false;
     unsigned long long *d_counts = nullptr;
+    unsigned long long *h_counts = nullptr;  // pinned, counts_cap entries
     long long counts_cap = 0;
 
     // phase timing (tb3_set_profiling)
@@ -633,6 +634,10 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
     const int per = c->nrep > 1 ? c->nrep : 1;
     if ((long long)steps * per > c->counts_cap) {
         dev_alloc(c->d_counts, (long long)steps * per);
+        if (c->h_counts) (void)hipHostFree(c->h_counts);
+        c->h_counts = nullptr;
+        HIPCHK(hipHostMalloc((void **)&c->h_counts, (size_t)steps * per * sizeof(unsigned long long),
+                             hipHostMallocDefault));
         c->counts_cap = (long long)steps * per;
     }
     int done = 0;
@@ -650,12 +655,17 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
         capture_step_graph(c, dt, g);
         launch_begin_steps(s, c->d_status, c->d_counts);
         for (int k = done; k < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec, s));
-        read_status(c, s);
+        // the status and every slot the replays could have recorded come back
+        // behind one synchronisation
+        HIPCHK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(Status), hipMemcpyDeviceToHost, s));
+        if (counts)
+            HIPCHK(hipMemcpyAsync(c->h_counts, c->d_counts, (size_t)(steps - done) * per * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
         const int ok = c->h_status->nstep;
         if (ok < 0 || ok > steps - done) throw Fail(TB3_EHIP, "replayed steps reported a bad step count");
         if (ok > 0 && counts)
-            HIPCHK(hipMemcpy(counts + (size_t)done * per, c->d_counts, (size_t)ok * per * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost));
+            std::memcpy(counts + (size_t)done * per, c->h_counts, (size_t)ok * per * sizeof(unsigned long long));
         done += ok;
         if (done < steps) {
             // step `done` overflowed: its predictor ran, its update did not
@@ -806,6 +816,7 @@ tb3_status tb3_destroy(tb3_ctx *c)
     for (hipEvent_t &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->h_counts) (void)hipHostFree(c->h_counts);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return TB3_OK;
