@@ -215,7 +215,7 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported",
+        print(f"bench.py: tb3 runs on one GPU; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported",
               file=sys.stderr)
         sys.exit(2)
 

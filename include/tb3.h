@@ -27,7 +27,7 @@ typedef enum {
     TB3_EINVAL = 1,       /* bad argument or call order */
     TB3_ENOMEM = 2,       /* device or host allocation failed */
     TB3_EHIP = 3,         /* a HIP runtime call failed */
-    TB3_EUNSUPPORTED = 4, /* valid request this build cannot serve */
+    TB3_EUNSUPPORTED = 4, /* valid request that tb3 does not serve */
     TB3_ENODEV = 5        /* no usable GPU */
 } tb3_status;
 

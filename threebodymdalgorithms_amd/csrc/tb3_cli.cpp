@@ -18,7 +18,7 @@
 // %g (6 significant digits) values, as Simulation.cpp:74-79, 99-133 and
 // utility.cpp:54-68.
 //
-// Extra flags of this build:
+// Extra flags (not in the reference):
 //   --precision exact|fp64|fp32   arithmetic of the sweep (default exact:
 //                                 bit-identical to the reference)
 //   --device N                    GPU ordinal
