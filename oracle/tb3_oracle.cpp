@@ -240,11 +240,14 @@ inline bool pair_ok(const double *pos, int a, int b, double cutoff, double c2, c
  * (Algorithm.cpp:51-140): j starts at i+1 when b1 and b0 have one owner,
  * else at 0 (77); k starts at j+1 when b2 and b1 have one owner, else at i+1
  * when b2 and b0 do, else at 0 (81-87); b0 rows [start, start + steps), or
- * every row when steps is -1 (72).  No cutoff: NATA and AUTA call the form
- * without one (Algorithm.cpp:23-29). */
+ * every row when steps is -1 (72).  NATA and AUTA call the form without a
+ * cutoff (Algorithm.cpp:23-36, cutoff -1); P3BCA passes its cutoff and the
+ * physical domain size (94-100). */
 uint64_t sweep_aos_owned(std::vector<Particle> &b0, std::vector<Particle> &b1, std::vector<Particle> &b2, int o0,
-                         int o1, int o2, int64_t start, int64_t steps, PotentialBase *pot)
+                         int o1, int o2, int64_t start, int64_t steps, PotentialBase *pot, double cutoff = -1.0,
+                         const double *L = nullptr)
 {
+    const double sqrCutoff = cutoff * cutoff;
     uint64_t act = 0;
     const size_t iend = steps != -1 ? (size_t)(start + steps) : b0.size();
     for (size_t i = (size_t)start; i < iend; ++i) {
@@ -259,6 +262,16 @@ uint64_t sweep_aos_owned(std::vector<Particle> &b0, std::vector<Particle> &b1, s
                 kstart = i + 1;
             for (size_t k = kstart; k < b2.size(); ++k) {
                 if (b2[k].isDummy) continue;
+                if (cutoff > 0) {
+                    if (sqr_dist_periodic(b0[i].pX, b0[i].pY, b0[i].pZ, b1[j].pX, b1[j].pY, b1[j].pZ, L) >
+                            sqrCutoff ||
+                        sqr_dist_periodic(b0[i].pX, b0[i].pY, b0[i].pZ, b2[k].pX, b2[k].pY, b2[k].pZ, L) >
+                            sqrCutoff ||
+                        sqr_dist_periodic(b1[j].pX, b1[j].pY, b1[j].pZ, b2[k].pX, b2[k].pY, b2[k].pZ, L) >
+                            sqrCutoff) {
+                        continue;
+                    }
+                }
                 pot->CalculateForces(b0[i], b1[j], b2[k]);
                 act++;
             }
@@ -306,6 +319,172 @@ inline bool same_triplet(const int *a, int b0, int b1, int b2)
     std::sort(y, y + 3);
     return x[0] == y[0] && x[1] == y[1] && x[2] == y[2];
 }
+
+/* The grid decomposition of a P3BCA run on p ranks along x (the reference's
+ * decompositions for 1..8 ranks): RegularGridDecomposition::Init
+ * (RegularGridDecomposition.cpp:6-39) with the domain of getDomainMinMax
+ * (135-166), cell width diff / (p, 1, 1), rank r's cell from min + r w to
+ * that plus w.  members[r] lists rank r's particles in buffer order. */
+struct P3bcaRanks {
+    int p = 1, nCbX = 0, numSteps = 1;
+    double L[3], lo[3], w[3], cmin[8][3], cmax[8][3];
+    std::vector<std::vector<int64_t>> members;
+};
+
+/* Init: binParticles keeps, in input order, the particles with
+ * min < x <= max in every dimension (isInsideLocalCell, 125-133); then
+ * P3BCA::Init's cutoff boxes and the validity check the reference exits on
+ * (P3BCA.cpp:7-83) and calcSteps in one dimension (85-98).  Returns 0 or -2. */
+int p3bca_ranks_init(P3bcaRanks &g, int p, int64_t n, const double *pos, double cutoff)
+{
+    g.p = p;
+    orc_domain_size(n, pos, g.L, g.lo);
+    g.w[0] = g.L[0] / (double)p;
+    g.w[1] = g.L[1] / 1.0;
+    g.w[2] = g.L[2] / 1.0;
+    g.members.assign((size_t)p, std::vector<int64_t>());
+    for (int r = 0; r < p; ++r) {
+        const double rk[3] = {(double)r, 0.0, 0.0};
+        for (int d = 0; d < 3; ++d) {
+            g.cmin[r][d] = g.lo[d] + rk[d] * g.w[d];
+            g.cmax[r][d] = g.cmin[r][d] + g.w[d];
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            const double *q = pos + 3 * i;
+            if (q[0] <= g.cmin[r][0] || q[1] <= g.cmin[r][1] || q[2] <= g.cmin[r][2] || q[0] > g.cmax[r][0] ||
+                q[1] > g.cmax[r][1] || q[2] > g.cmax[r][2])
+                continue;
+            g.members[(size_t)r].push_back(i);
+        }
+    }
+    g.nCbX = p == 1 ? 0 : (p == 2 ? 1 : (int)std::ceil(cutoff / g.w[0]));
+    if (p > 2 && (g.nCbX < 1 || (double)g.nCbX >= (double)p / 2.0)) return -2;
+    g.numSteps = g.nCbX + 1;
+    return 0;
+}
+
+/* RegularGridDecomposition::Update after the integrator (168-177):
+ * exchangeParticlesDim along x (59-123).  Each rank sends, in buffer order,
+ * the particles left of its cell to the left neighbour and those right of it
+ * to the right neighbour (periodic ranks), keeps the rest in order, and
+ * appends what it receives from the left, then from the right. */
+void p3bca_ranks_exchange(P3bcaRanks &g, const double *pos)
+{
+    const int p = g.p;
+    if (p <= 1) return;
+    std::vector<std::vector<int64_t>> toL((size_t)p), toR((size_t)p);
+    for (int r = 0; r < p; ++r) {
+        std::vector<int64_t> keep;
+        for (int64_t i : g.members[(size_t)r]) {
+            const double x = pos[3 * i];
+            if (x < g.cmin[r][0]) toL[(size_t)r].push_back(i);
+            else if (x > g.cmax[r][0]) toR[(size_t)r].push_back(i);
+            else keep.push_back(i);
+        }
+        g.members[(size_t)r] = keep;
+    }
+    for (int r = 0; r < p; ++r) {
+        const int left = ring_mod(r - 1, p), right = ring_mod(r + 1, p);
+        for (int64_t i : toR[(size_t)left]) g.members[(size_t)r].push_back(i);
+        for (int64_t i : toL[(size_t)right]) g.members[(size_t)r].push_back(i);
+    }
+}
+
+/* P3BCA::SimulationStep (P3BCA.cpp:516-805) over the current members. */
+void p3bca_ranks_sweep(const P3bcaRanks &g, int64_t n, const double *pos, double nu, double cutoff, double *f,
+                       uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed, int64_t cap,
+                       int64_t *nprocessed)
+{
+    const int p = g.p, numSteps = g.numSteps;
+    const double *L = g.L;
+    const std::vector<std::vector<int64_t>> &members = g.members;
+    std::vector<std::vector<Particle>> own((size_t)p), c1((size_t)p), c2((size_t)p);
+    for (int r = 0; r < p; ++r) {
+        std::vector<Particle> b((size_t)members[(size_t)r].size());
+        for (size_t t = 0; t < b.size(); ++t) {
+            const int64_t i = members[(size_t)r][t];
+            std::memset(&b[t], 0, sizeof(Particle));
+            b[t].ID = (int)i;
+            b[t].pX = pos[3 * i];
+            b[t].pY = pos[3 * i + 1];
+            b[t].pZ = pos[3 * i + 2];
+        }
+        own[(size_t)r] = b;
+        c1[(size_t)r] = b;
+        c2[(size_t)r] = b;
+    }
+    AxilrodTellerOracle at(nu);
+    g_potential_slot = &at;
+    /* SimulationStep (516-805): b1 and b2 start as copies of b0 and travel;
+     * shiftLeft (321-335) hands rank r the buffer that the rank at
+     * (next source + offset vector) held, then adds the step's difference
+     * to the offset vector (handleOffsetVector1D, 311-319).  Every rank runs
+     * the same loop, so the ranks move in lockstep. */
+    std::vector<int> hold1((size_t)p), hold2((size_t)p), nsi((size_t)p), dfi((size_t)p), nso((size_t)p),
+        dfo((size_t)p), ovi((size_t)p, 0), ovo((size_t)p, 0);
+    for (int r = 0; r < p; ++r) hold1[(size_t)r] = hold2[(size_t)r] = r;
+    auto shift = [&](std::vector<int> &hold, const std::vector<int> &src, std::vector<int> &ov,
+                     const std::vector<int> &df) {
+        const std::vector<int> old = hold;
+        for (int r = 0; r < p; ++r) {
+            hold[(size_t)r] = old[(size_t)ring_mod(src[(size_t)r] + ov[(size_t)r], p)];
+            ov[(size_t)r] += df[(size_t)r];
+        }
+    };
+    for (int r = 0; r < p; ++r) buffer_counts[r] = particle_counts[r] = 0;
+    int64_t np = 0;
+    const int outer = p == 2 ? 1 : numSteps; /* the edge case of two ranks (577) */
+    for (int i2 = 0; i2 < outer; ++i2) {
+        for (int i3 = i2; i3 < numSteps; ++i3) {
+            for (int r = 0; r < p; ++r) {
+                const int o1 = hold1[(size_t)r], o2 = hold2[(size_t)r];
+                particle_counts[r] += sweep_aos_owned(own[(size_t)r], c1[(size_t)o1], c2[(size_t)o2], r, o1, o2, 0, -1,
+                                                      g_potential_slot, cutoff, L);
+                buffer_counts[r]++;
+                if (processed && np < cap) {
+                    processed[4 * np] = r;
+                    processed[4 * np + 1] = r;
+                    processed[4 * np + 2] = o1;
+                    processed[4 * np + 3] = o2;
+                }
+                ++np;
+            }
+            if (i3 < numSteps - 1) {
+                /* schedule1DHelper (110-118) with calcDiff1D (120-128) */
+                for (int r = 0; r < p; ++r) {
+                    nsi[(size_t)r] = ring_mod((i3 + 1) + r, p);
+                    dfi[(size_t)r] = ring_mod(i3 + r, p) - nsi[(size_t)r];
+                }
+                shift(hold2, nsi, ovi, dfi);
+            }
+        }
+        if (i2 < outer - 1) {
+            /* schedule1D (100) and calcDiff1D for both buffers; b1 moves with
+             * the outer offset vector, b2 with the inner one, which then takes
+             * the outer one's value */
+            for (int r = 0; r < p; ++r) {
+                nso[(size_t)r] = nsi[(size_t)r] = ring_mod((i2 + 1) + r, p);
+                dfo[(size_t)r] = dfi[(size_t)r] = ring_mod(i2 + r, p) - nso[(size_t)r];
+            }
+            shift(hold1, nso, ovo, dfo);
+            shift(hold2, nsi, ovi, dfi);
+            ovi = ovo;
+        }
+    }
+    /* sendBackParticles (415-504), then SumUpParticles on each rank's own
+     * buffers; particles no rank binned keep a zero force */
+    for (int64_t i = 0; i < 3 * n; ++i) f[i] = 0.0;
+    for (int q = 0; q < p; ++q)
+        for (size_t t = 0; t < members[(size_t)q].size(); ++t) {
+            const Particle &a = own[(size_t)q][t], &b = c1[(size_t)q][t], &c = c2[(size_t)q][t];
+            double *fp = f + 3 * members[(size_t)q][t];
+            fp[0] = a.fX + (b.fX + c.fX);
+            fp[1] = a.fY + (b.fY + c.fY);
+            fp[2] = a.fZ + (b.fZ + c.fZ);
+        }
+    if (nprocessed) *nprocessed = np;
+}
+
 
 }  // namespace
 
@@ -856,6 +1035,37 @@ int orc_sweep_ranks(int algo, int ranks, int64_t n, const double *pos, double nu
             fp[2] = a.fZ + (b.fZ + c.fZ);
         }
     if (nprocessed) *nprocessed = np;
+    return 0;
+}
+
+int orc_sweep_ranks_p3bca(int ranks, int64_t n, const double *pos, double nu, double cutoff, double *f,
+                          uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed, int64_t cap,
+                          int64_t *nprocessed)
+{
+    if (ranks < 1 || ranks > 8 || n < 0 || !(cutoff > 0)) return -1;
+    P3bcaRanks g;
+    if (p3bca_ranks_init(g, ranks, n, pos, cutoff) != 0) return -2;
+    p3bca_ranks_sweep(g, n, pos, nu, cutoff, f, buffer_counts, particle_counts, processed, cap, nprocessed);
+    return 0;
+}
+
+int orc_simulate_ranks_p3bca(int ranks, int64_t n, double *pos, double *vel, double *acc, const double *mass,
+                             double *f, int steps, double dt, const double *g, double cutoff, double nu,
+                             uint64_t *buffer_counts, uint64_t *particle_counts, orc_step_cb step_cb, void *user)
+{
+    if (ranks < 1 || ranks > 8 || n < 0 || !(cutoff > 0)) return -1;
+    /* Simulation::Init bins the initial particles; Start then runs predictor,
+     * sweep and update (with the exchange) per step (Simulation.cpp:28-81) */
+    P3bcaRanks grid;
+    if (p3bca_ranks_init(grid, ranks, n, pos, cutoff) != 0) return -2;
+    for (int s = 0; s < steps; ++s) {
+        orc_predictor(n, pos, vel, acc, dt);
+        p3bca_ranks_sweep(grid, n, pos, nu, cutoff, f, buffer_counts + (size_t)s * ranks,
+                          particle_counts + (size_t)s * ranks, nullptr, 0, nullptr);
+        orc_update(n, f, mass, vel, acc, dt, g);
+        p3bca_ranks_exchange(grid, pos);
+        if (step_cb) step_cb(s, user);
+    }
     return 0;
 }
 

@@ -13,9 +13,10 @@
  *                               (the MEASURESIMSTEP_3BMDA line, Simulation.cpp:50-72)
  *   --counts                    print the accepted-triplet count of each step
  *   --full-precision            write CSV values with %.17g instead of %g
- *   --ranks P                   run NATA or AUTA as `mpiexec -n P` does (the
- *                               multi-rank sweep of orc_sweep_ranks); --counts
- *                               then prints each rank's two counts
+ *   --ranks P                   run as `mpiexec -n P` does: NATA and AUTA on
+ *                               any P (orc_sweep_ranks), P3BCA on 1..8
+ *                               (orc_simulate_ranks_p3bca); --counts then
+ *                               prints each rank's two counts per step
  */
 #include <algorithm>
 #include <chrono>
@@ -210,8 +211,26 @@ int main(int argc, char **argv)
         /* Simulation::Start with the multi-rank sweep: the integrator and the
          * CSV writer act per particle, so only the sweep and its counts differ */
         if (a.algorithm == 1) {
-            std::cerr << "--ranks: P3BCA's multi-rank domain decomposition is not restated" << std::endl;
-            return 1;
+            const int it = std::max(a.iterations, 0);
+            std::vector<uint64_t> bcs((size_t)std::max(it, 1) * a.ranks), pcs(bcs.size());
+            const int rc = orc_simulate_ranks_p3bca(a.ranks, n, t.p.data(), t.v.data(), t.acc.data(), t.m.data(),
+                                                    t.f.data(), it, a.deltaT, a.g, a.cutoff, 1.0, bcs.data(),
+                                                    pcs.data(), stepCallback, &st);
+            if (rc == -2) {
+                std::cerr << "invalid cutoff: " << a.cutoff << " for " << a.ranks << " ranks" << std::endl;
+                return 1;
+            }
+            if (rc != 0) {
+                std::cerr << "--ranks: P3BCA runs on 1 to 8 ranks with a positive cutoff" << std::endl;
+                return 1;
+            }
+            if (a.counts)
+                for (int s = 0; s < it; ++s)
+                    for (int r = 0; r < a.ranks; ++r)
+                        std::cout << "step " << s << ": proc " << r << ": " << pcs[(size_t)s * a.ranks + r]
+                                  << " particle interactions, " << bcs[(size_t)s * a.ranks + r]
+                                  << " buffer interactions" << std::endl;
+            return 0;
         }
         std::vector<uint64_t> bc((size_t)a.ranks), pc((size_t)a.ranks);
         for (int s = 0; s < a.iterations; ++s) {

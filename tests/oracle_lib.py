@@ -66,6 +66,14 @@ def lib():
                                       _u64p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_int64)]
         L.orc_sweep_ranks.restype = ctypes.c_int
+        L.orc_sweep_ranks_p3bca.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp,
+                                            _u64p, _u64p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_int64)]
+        L.orc_sweep_ranks_p3bca.restype = ctypes.c_int
+        L.orc_simulate_ranks_p3bca.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                               ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double, _u64p, _u64p,
+                                               ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_simulate_ranks_p3bca.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -104,9 +112,9 @@ def sweep(pos, nu=1.0, cutoff=-1.0, L=None, mode="owner", threads=0):
     return f0, f1, f2, f, int(cnt)
 
 
-def sweep_ranks(pos, algo, ranks, nu=1.0):
-    """The step sweep of NATA or AUTA as `mpiexec -n ranks` runs it, restated
-    in one process: returns (f, buffer_counts, particle_counts, processed)
+def sweep_ranks(pos, algo, ranks, nu=1.0, cutoff=-1.0):
+    """The step sweep of NATA, AUTA or P3BCA (with `cutoff`) as `mpiexec -n
+    ranks` runs it, restated in one process: returns (f, buffer_counts, particle_counts, processed)
     with processed a (k, 4) array of (rank, owner of b0, b1, b2) rows."""
     pos = _c(pos)
     n = pos.shape[0]
@@ -116,13 +124,35 @@ def sweep_ranks(pos, algo, ranks, nu=1.0):
     cap = 4 * (ranks + 2) ** 3
     proc = np.zeros((cap, 4), dtype=np.int32)
     nproc = ctypes.c_int64(0)
-    rc = lib().orc_sweep_ranks(ALGO[algo], ranks, n, _p(pos), nu, _p(f), bc.ctypes.data_as(_u64p),
-                               pc.ctypes.data_as(_u64p), proc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap,
-                               ctypes.byref(nproc))
+    if algo == "p3bca":
+        rc = lib().orc_sweep_ranks_p3bca(ranks, n, _p(pos), nu, cutoff, _p(f), bc.ctypes.data_as(_u64p),
+                                         pc.ctypes.data_as(_u64p), proc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                         cap, ctypes.byref(nproc))
+    else:
+        rc = lib().orc_sweep_ranks(ALGO[algo], ranks, n, _p(pos), nu, _p(f), bc.ctypes.data_as(_u64p),
+                                   pc.ctypes.data_as(_u64p), proc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap,
+                                   ctypes.byref(nproc))
     if rc != 0:
-        raise ValueError(f"orc_sweep_ranks({algo}, {ranks}) returned {rc}")
+        raise ValueError(f"the {algo} sweep at {ranks} ranks returned {rc}")
     assert nproc.value <= cap
     return f, bc, pc, proc[:nproc.value]
+
+
+def simulate_ranks_p3bca(pos, vel, acc, mass, ranks, steps, dt, g, cutoff, nu=1.0):
+    """A P3BCA run on `ranks` ranks (1..8) with the particle exchange between
+    steps: returns (pos, vel, acc, f, buffer_counts, particle_counts), the
+    counts as (steps, ranks) arrays."""
+    pos, vel, acc = (_c(a).copy() for a in (pos, vel, acc))
+    n = pos.shape[0]
+    f = np.zeros((n, 3))
+    bc = np.zeros((steps, ranks), dtype=np.uint64)
+    pc = np.zeros((steps, ranks), dtype=np.uint64)
+    rc = lib().orc_simulate_ranks_p3bca(ranks, n, _p(pos), _p(vel), _p(acc), _p(_c(mass)), _p(f), steps, dt,
+                                        _p(_c(g)), cutoff, nu, bc.ctypes.data_as(_u64p), pc.ctypes.data_as(_u64p),
+                                        None, None)
+    if rc != 0:
+        raise ValueError(f"the p3bca run at {ranks} ranks returned {rc}")
+    return pos, vel, acc, f, bc, pc
 
 
 def brute_possible(pos, cutoff=-1.0, L=None, nu=1.0):

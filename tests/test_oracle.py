@@ -372,14 +372,22 @@ def test_multi_rank_sweep(oracle, algo, ranks):
 
 
 def test_multi_rank_unsupported(oracle):
+    """P3BCA needs a positive cutoff, and only its 1-D decompositions (1..8
+    ranks) are restated."""
+    P = uniform_particles(10, 3.0, 1)
     with pytest.raises(ValueError):
-        oracle.sweep_ranks(uniform_particles(10, 3.0, 1), "p3bca", 2)
+        oracle.sweep_ranks(P, "p3bca", 2)
+    with pytest.raises(ValueError):
+        oracle.sweep_ranks(P, "p3bca", 9, cutoff=1.0)
 
 
-def test_oracle_cli_ranks_matches_one_process(oracle, tmp_path):
-    """`--ranks 4` runs the same trajectory as one process up to the
-    reordered sums, and prints each rank's two counts, whose particle counts
-    add up to C(N,3) per step."""
+@pytest.mark.parametrize("algo,ranks,cutoff", [("nata", 4, -1.0), ("p3bca", 2, 1.0)])
+def test_oracle_cli_ranks_matches_one_process(oracle, tmp_path, algo, ranks, cutoff):
+    """`--ranks` runs the same trajectory as one process up to the reordered
+    sums, and prints each rank's two counts, whose particle counts add up to
+    the one-process count per step (C(N,3) for NATA; on two ranks P3BCA
+    computes every rank triplet, so its counts match whatever the particles
+    do)."""
     from math import comb
     n = 30
     P = uniform_particles(n, 3.0, 8)
@@ -388,21 +396,118 @@ def test_oracle_cli_ranks_matches_one_process(oracle, tmp_path):
         fh.write("ID, pX, pY, pZ, vX, vY, vZ, aX, aY, aZ, m\n")
         for i in range(n):
             fh.write(f"{i}, {P[i,0]:.17g}, {P[i,1]:.17g}, {P[i,2]:.17g}, 0, 0, 0, 0, 0, 0, 1\n")
-    outs = {}
-    for ranks in (1, 4):
-        out = tmp_path / f"r{ranks}.csv"
-        r = subprocess.run([oracle.ORACLE_CLI, "-a", "nata", "-i", "2", "-d", "0.001", "-gy", "-9.81", "-csv",
-                            str(src), "-o", str(out), "--full-precision", "--ranks", str(ranks), "--counts"],
-                           capture_output=True, text=True)
+    outs, counts = {}, {}
+    for p in (1, ranks):
+        out = tmp_path / f"r{p}.csv"
+        cmd = [oracle.ORACLE_CLI, "-a", algo, "-i", "2", "-d", "0.001", "-gy", "-9.81", "-csv", str(src), "-o",
+               str(out), "--full-precision", "--ranks", str(p), "--counts"]
+        if cutoff > 0:
+            cmd += ["-c", str(cutoff)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
-        outs[ranks] = np.loadtxt(tmp_path / f"r{ranks}_1.csv", delimiter=",", skiprows=1)
-        if ranks == 4:
-            per_step = {}
-            for line in r.stdout.splitlines():
-                if line.startswith("step ") and "proc" in line:
-                    s = int(line.split()[1].rstrip(":"))
-                    per_step[s] = per_step.get(s, 0) + int(line.split(": ")[2].split()[0])
-            assert per_step == {0: comb(n, 3), 1: comb(n, 3)}
-    a, b = outs[4], outs[1]
+        outs[p] = np.loadtxt(tmp_path / f"r{p}_1.csv", delimiter=",", skiprows=1)
+        per_step = {}
+        for line in r.stdout.splitlines():
+            if line.startswith("step ") and "particle interactions" in line:
+                s = int(line.split()[1].rstrip(":"))
+                per_step[s] = per_step.get(s, 0) + int(line.split("particle interactions")[0].split()[-1])
+        counts[p] = per_step
+    assert counts[ranks] == counts[1]
+    if algo == "nata":
+        assert counts[1] == {0: comb(n, 3), 1: comb(n, 3)}
+    a, b = outs[ranks], outs[1]
     assert np.array_equal(a[:, 0], b[:, 0])
     assert np.allclose(a[:, 1:], b[:, 1:], rtol=1e-11, atol=1e-11)
+
+
+def _p3bca_expected_1d(p, ncb):
+    """The reference test's expected buffer triplets per rank (tests.cpp:276-335
+    with customLt 254-259 and periodicDiff 243-247) on a 1-D decomposition of
+    p ranks with ncb cutoff boxes."""
+    def pdist(x, y):
+        return min(abs(x - y), p - abs(x - y))
+
+    def pdiff(x, y):
+        return (x - y) if abs(x - y) <= p // 2 else (1 if y - x >= 0 else -1) * pdist(x, y)
+
+    def lt(r, u, v):  # vLtV on 1-D differences: true when less or equal
+        return pdiff(u, r) <= pdiff(v, r)
+
+    win = {r: [q for q in range(p) if pdist(q, r) <= ncb] for r in range(p)}
+    out = {}
+    for r in range(p):
+        out[r] = [(r, i2, i3) for i2 in win[r] if lt(r, r, i2) for i3 in win[r] if i3 in win[i2] and lt(r, i2, i3)]
+    return out
+
+
+@pytest.mark.parametrize("ranks,cutoff", [(1, 2.0), (2, 2.0), (3, 2.0), (4, 2.0), (5, 2.0), (6, 2.0), (7, 2.0),
+                                          (8, 2.0), (8, 1.5), (5, 1.2)])
+def test_multi_rank_sweep_p3bca(oracle, ranks, cutoff):
+    """P3BCA's step sweep as `mpiexec -n ranks` runs it (1-D decompositions,
+    1..8 ranks), restated in one process.  Pinned by:
+      * the reference's coverage test (tests.cpp:691-825): each rank's buffer
+        triplets equal the expected set of its cutoff-box window
+        (tests.cpp:276-335), with no triplet expected twice, on one rank or
+        across ranks.  At two ranks the step's special case (P3BCA.cpp:577)
+        computes the same triplets split differently between the ranks, so
+        there the union is compared;
+      * every accepted particle triplet once: the counts sum to the
+        one-process count;
+      * the forces equal the one-process sweep up to reordering, bit for bit
+        at p = 1."""
+    from collections import Counter
+    n = 1500
+    box = (n / 2.0) ** (1 / 3) * 1.6
+    P = uniform_particles(n, box, 70 + ranks)
+    L, _ = oracle.domain_size(P)
+    *_, fo, co = oracle.sweep(P, cutoff=cutoff, L=L, mode="owner")
+    f, bc, pc, proc = oracle.sweep_ranks(P, "p3bca", ranks, cutoff=cutoff)
+    assert int(pc.sum()) == co
+    ncb = 0 if ranks == 1 else (1 if ranks == 2 else int(np.ceil(cutoff / (L[0] / ranks))))
+    exp = _p3bca_expected_1d(ranks, ncb)
+    every = [tuple(sorted(t)) for r in exp for t in exp[r]]
+    assert len(every) == len(set(every))
+    got_all = Counter(tuple(sorted(int(x) for x in row[1:])) for row in proc)
+    assert got_all == Counter(every)
+    if ranks != 2:  # two ranks take the reference's special case: the same triplets, split differently
+        for r in range(ranks):
+            got = Counter(tuple(sorted(int(x) for x in row[1:])) for row in proc if row[0] == r)
+            assert got == Counter(tuple(sorted(t)) for t in exp[r]), r
+    if ranks == 1:
+        assert np.array_equal(f, fo)
+    else:
+        num = np.linalg.norm(f - fo, axis=1)
+        den = np.linalg.norm(fo, axis=1)
+        assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max() <= 1e-12
+
+
+def test_multi_rank_p3bca_invalid_cutoff(oracle):
+    """More than two ranks and a cutoff needing at least ranks / 2 cutoff
+    boxes: the reference exits (P3BCA.cpp:55-60); the oracle reports it."""
+    P = uniform_particles(200, 8.0, 3)
+    with pytest.raises(ValueError):
+        oracle.sweep_ranks(P, "p3bca", 4, cutoff=5.0)
+
+
+@pytest.mark.parametrize("ranks", [2, 3, 4, 5, 6])
+def test_multi_rank_p3bca_run_with_exchange(oracle, ranks):
+    """Six P3BCA steps on several ranks, with three fast particles crossing
+    the ranks' x-slabs inside the domain, so the particle exchange between
+    steps (RegularGridDecomposition::Update) moves them from rank to rank:
+    every step's counts equal the one-process run's, and the state matches it
+    to the reordering of the force sums."""
+    rng = np.random.default_rng(2)
+    m, sp = 10, 1.2
+    grid = np.stack(np.meshgrid(*[np.arange(m) * sp] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.3
+    P = grid + rng.normal(0, 0.05, grid.shape)
+    n = len(P)
+    V, A, M = np.zeros_like(P), np.zeros_like(P), np.ones(n)
+    V[7, 0], V[500, 0], V[250, 0] = 600.0, -700.0, 450.0
+    g = np.array([0.0, -9.81, 0.0])
+    pos1, vel1, acc1, f1, oc = oracle.simulate(P, V, A, M, "p3bca", 6, 0.001, g, cutoff=1.5)
+    pos, vel, acc, f, bc, pc = oracle.simulate_ranks_p3bca(P, V, A, M, ranks, 6, 0.001, g, 1.5)
+    assert [int(c) for c in pc.sum(axis=1)] == [int(c) for c in oc]
+    assert np.abs(pos - pos1).max() <= 1e-12
+    num = np.linalg.norm(f - f1, axis=1)
+    den = np.linalg.norm(f1, axis=1)
+    assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max() <= 1e-12
