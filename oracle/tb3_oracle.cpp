@@ -320,78 +320,125 @@ inline bool same_triplet(const int *a, int b0, int b1, int b2)
     return x[0] == y[0] && x[1] == y[1] && x[2] == y[2];
 }
 
-/* The grid decomposition of a P3BCA run on p ranks along x (the reference's
- * decompositions for 1..8 ranks): RegularGridDecomposition::Init
+/* The grid decomposition of a P3BCA run on a Cartesian decomposition dims
+ * (1 to 3 dimensions, periodic; ranks numbered row-major as MPI_Cart_create
+ * numbers them without reordering): RegularGridDecomposition::Init
  * (RegularGridDecomposition.cpp:6-39) with the domain of getDomainMinMax
- * (135-166), cell width diff / (p, 1, 1), rank r's cell from min + r w to
+ * (135-166), cell widths diff / dims, rank r's cell from min + coords w to
  * that plus w.  members[r] lists rank r's particles in buffer order. */
 struct P3bcaRanks {
-    int p = 1, nCbX = 0, numSteps = 1;
-    double L[3], lo[3], w[3], cmin[8][3], cmax[8][3];
+    int p = 1, nd = 1, dims[3] = {1, 1, 1}, ncb[3] = {0, 0, 0}, numSteps = 1;
+    double L[3], lo[3], w[3];
+    std::vector<std::array<double, 3>> cmin, cmax;
     std::vector<std::vector<int64_t>> members;
+
+    std::array<int, 3> coords(int r) const
+    {
+        return {r / (dims[1] * dims[2]), (r / dims[2]) % dims[1], r % dims[2]};
+    }
+    int rank(const std::array<int, 3> &c) const { return (c[0] * dims[1] + c[1]) * dims[2] + c[2]; }
+    /* schedule1D / schedule2D / schedule3D (P3BCA.cpp:100, 130-139, 183-195):
+     * the i-th source of the rank at c */
+    std::array<int, 3> sched(int i, const std::array<int, 3> &c) const
+    {
+        if (nd == 1) return {ring_mod(i + c[0], dims[0]), c[1], c[2]};
+        const int lenY = 2 * ncb[1] + 1;
+        if (nd == 2) {
+            const int gy = ((i + ncb[1]) % lenY) - ncb[1];
+            const int gx = (i + ncb[1]) / lenY;
+            return {ring_mod(gx + c[0], dims[0]), ring_mod(gy + c[1], dims[1]), c[2]};
+        }
+        const int lenZ = 2 * ncb[2] + 1;
+        const int gz = ((i + ncb[2]) % lenZ) - ncb[2];
+        const int gy = ((((i + ncb[2]) / lenZ) + ncb[1]) % lenY) - ncb[1];
+        const int gx = (i + ((lenZ * lenY) / 2)) / (lenZ * lenY);
+        return {ring_mod(gx + c[0], dims[0]), ring_mod(gy + c[1], dims[1]), ring_mod(gz + c[2], dims[2])};
+    }
+    /* calcDiff1D/2D/3D (120-128, 209-245): the (i-1)-th source minus src,
+     * component by component, unreduced */
+    std::array<int, 3> diff(const std::array<int, 3> &c, const std::array<int, 3> &src, int i) const
+    {
+        const std::array<int, 3> old = sched(i - 1, c);
+        return {old[0] - src[0], old[1] - src[1], old[2] - src[2]};
+    }
 };
 
 /* Init: binParticles keeps, in input order, the particles with
  * min < x <= max in every dimension (isInsideLocalCell, 125-133); then
  * P3BCA::Init's cutoff boxes and the validity check the reference exits on
- * (P3BCA.cpp:7-83) and calcSteps in one dimension (85-98).  Returns 0 or -2. */
-int p3bca_ranks_init(P3bcaRanks &g, int p, int64_t n, const double *pos, double cutoff)
+ * (P3BCA.cpp:7-83), and calcSteps (85-98).  Returns 0 or -2. */
+int p3bca_ranks_init(P3bcaRanks &g, int nd, const int *dims, int64_t n, const double *pos, double cutoff)
 {
-    g.p = p;
+    g.nd = nd;
+    g.p = 1;
+    for (int k = 0; k < 3; ++k) {
+        g.dims[k] = k < nd ? dims[k] : 1;
+        g.p *= g.dims[k];
+    }
     orc_domain_size(n, pos, g.L, g.lo);
-    g.w[0] = g.L[0] / (double)p;
-    g.w[1] = g.L[1] / 1.0;
-    g.w[2] = g.L[2] / 1.0;
-    g.members.assign((size_t)p, std::vector<int64_t>());
-    for (int r = 0; r < p; ++r) {
-        const double rk[3] = {(double)r, 0.0, 0.0};
-        for (int d = 0; d < 3; ++d) {
-            g.cmin[r][d] = g.lo[d] + rk[d] * g.w[d];
-            g.cmax[r][d] = g.cmin[r][d] + g.w[d];
+    for (int k = 0; k < 3; ++k) g.w[k] = g.L[k] / (double)g.dims[k];
+    g.members.assign((size_t)g.p, std::vector<int64_t>());
+    g.cmin.resize((size_t)g.p);
+    g.cmax.resize((size_t)g.p);
+    for (int r = 0; r < g.p; ++r) {
+        const std::array<int, 3> c = g.coords(r);
+        for (int k = 0; k < 3; ++k) {
+            g.cmin[(size_t)r][k] = g.lo[k] + (double)c[k] * g.w[k];
+            g.cmax[(size_t)r][k] = g.cmin[(size_t)r][k] + g.w[k];
         }
+        const std::array<double, 3> &mn = g.cmin[(size_t)r], &mx = g.cmax[(size_t)r];
         for (int64_t i = 0; i < n; ++i) {
             const double *q = pos + 3 * i;
-            if (q[0] <= g.cmin[r][0] || q[1] <= g.cmin[r][1] || q[2] <= g.cmin[r][2] || q[0] > g.cmax[r][0] ||
-                q[1] > g.cmax[r][1] || q[2] > g.cmax[r][2])
+            if (q[0] <= mn[0] || q[1] <= mn[1] || q[2] <= mn[2] || q[0] > mx[0] || q[1] > mx[1] || q[2] > mx[2])
                 continue;
             g.members[(size_t)r].push_back(i);
         }
     }
-    g.nCbX = p == 1 ? 0 : (p == 2 ? 1 : (int)std::ceil(cutoff / g.w[0]));
-    if (p > 2 && (g.nCbX < 1 || (double)g.nCbX >= (double)p / 2.0)) return -2;
-    g.numSteps = g.nCbX + 1;
+    g.ncb[0] = g.p == 1 ? 0 : (g.p == 2 ? 1 : (int)std::ceil(cutoff / g.w[0]));
+    g.ncb[1] = nd > 1 ? (int)std::ceil(cutoff / g.w[1]) : 0;
+    g.ncb[2] = nd > 2 ? (int)std::ceil(cutoff / g.w[2]) : 0;
+    if (g.p > 2 && (g.ncb[0] < 1 || (double)g.ncb[0] >= (double)g.dims[0] / 2.0)) return -2;
+    if (nd > 1 && (g.ncb[1] < 1 || (double)g.ncb[1] >= (double)g.dims[1] / 2.0)) return -2;
+    if (nd > 2 && (g.ncb[2] < 1 || (double)g.ncb[2] >= (double)g.dims[2] / 2.0)) return -2;
+    const int x = g.ncb[0], y = g.ncb[1], z = g.ncb[2];
+    g.numSteps = nd == 1 ? x + 1 : (nd == 2 ? 2 * x * y + x + y + 1 : 1 + z + y * (z * 2 + 1) + x * ((y * 2 + 1) * (z * 2 + 1)));
     return 0;
 }
 
 /* RegularGridDecomposition::Update after the integrator (168-177):
- * exchangeParticlesDim along x (59-123).  Each rank sends, in buffer order,
- * the particles left of its cell to the left neighbour and those right of it
- * to the right neighbour (periodic ranks), keeps the rest in order, and
- * appends what it receives from the left, then from the right. */
+ * exchangeParticlesDim for each dimension in turn (59-123).  Each rank
+ * sends, in buffer order, the particles below its cell to the lower
+ * neighbour and those above it to the upper neighbour (periodic ranks),
+ * keeps the rest in order, and appends what it receives from the lower
+ * neighbour, then from the upper one. */
 void p3bca_ranks_exchange(P3bcaRanks &g, const double *pos)
 {
     const int p = g.p;
     if (p <= 1) return;
-    std::vector<std::vector<int64_t>> toL((size_t)p), toR((size_t)p);
-    for (int r = 0; r < p; ++r) {
-        std::vector<int64_t> keep;
-        for (int64_t i : g.members[(size_t)r]) {
-            const double x = pos[3 * i];
-            if (x < g.cmin[r][0]) toL[(size_t)r].push_back(i);
-            else if (x > g.cmax[r][0]) toR[(size_t)r].push_back(i);
-            else keep.push_back(i);
+    for (int d = 0; d < g.nd; ++d) {
+        std::vector<std::vector<int64_t>> toL((size_t)p), toR((size_t)p);
+        for (int r = 0; r < p; ++r) {
+            std::vector<int64_t> keep;
+            for (int64_t i : g.members[(size_t)r]) {
+                const double x = pos[3 * i + d];
+                if (x < g.cmin[(size_t)r][d]) toL[(size_t)r].push_back(i);
+                else if (x > g.cmax[(size_t)r][d]) toR[(size_t)r].push_back(i);
+                else keep.push_back(i);
+            }
+            g.members[(size_t)r] = keep;
         }
-        g.members[(size_t)r] = keep;
-    }
-    for (int r = 0; r < p; ++r) {
-        const int left = ring_mod(r - 1, p), right = ring_mod(r + 1, p);
-        for (int64_t i : toR[(size_t)left]) g.members[(size_t)r].push_back(i);
-        for (int64_t i : toL[(size_t)right]) g.members[(size_t)r].push_back(i);
+        for (int r = 0; r < p; ++r) {
+            std::array<int, 3> lc = g.coords(r), rc = lc;
+            lc[d] = ring_mod(lc[d] - 1, g.dims[d]);
+            rc[d] = ring_mod(rc[d] + 1, g.dims[d]);
+            for (int64_t i : toR[(size_t)g.rank(lc)]) g.members[(size_t)r].push_back(i);
+            for (int64_t i : toL[(size_t)g.rank(rc)]) g.members[(size_t)r].push_back(i);
+        }
     }
 }
 
 /* P3BCA::SimulationStep (P3BCA.cpp:516-805) over the current members. */
-void p3bca_ranks_sweep(const P3bcaRanks &g, int64_t n, const double *pos, double nu, double cutoff, double *f,
+int p3bca_ranks_sweep(const P3bcaRanks &g, int64_t n, const double *pos, double nu, double cutoff, double *f,
                        uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed, int64_t cap,
                        int64_t *nprocessed)
 {
@@ -416,19 +463,23 @@ void p3bca_ranks_sweep(const P3bcaRanks &g, int64_t n, const double *pos, double
     AxilrodTellerOracle at(nu);
     g_potential_slot = &at;
     /* SimulationStep (516-805): b1 and b2 start as copies of b0 and travel;
-     * shiftLeft (321-335) hands rank r the buffer that the rank at
+     * shiftLeft (321-367) hands rank r the buffer that the rank at
      * (next source + offset vector) held, then adds the step's difference
-     * to the offset vector (handleOffsetVector1D, 311-319).  Every rank runs
-     * the same loop, so the ranks move in lockstep. */
-    std::vector<int> hold1((size_t)p), hold2((size_t)p), nsi((size_t)p), dfi((size_t)p), nso((size_t)p),
-        dfo((size_t)p), ovi((size_t)p, 0), ovo((size_t)p, 0);
+     * to the offset vector (handleOffsetVector*, 278-319).  Every rank runs
+     * the same loop; the 2-D and 3-D helpers may skip inner steps, by an
+     * amount that does not depend on the rank. */
+    using A3 = std::array<int, 3>;
+    std::vector<int> hold1((size_t)p), hold2((size_t)p);
+    std::vector<A3> nsi((size_t)p), dfi((size_t)p), nso((size_t)p), dfo((size_t)p), ovi((size_t)p, A3{0, 0, 0}),
+        ovo((size_t)p, A3{0, 0, 0});
     for (int r = 0; r < p; ++r) hold1[(size_t)r] = hold2[(size_t)r] = r;
-    auto shift = [&](std::vector<int> &hold, const std::vector<int> &src, std::vector<int> &ov,
-                     const std::vector<int> &df) {
+    auto shift = [&](std::vector<int> &hold, const std::vector<A3> &src, std::vector<A3> &ov, const std::vector<A3> &df) {
         const std::vector<int> old = hold;
         for (int r = 0; r < p; ++r) {
-            hold[(size_t)r] = old[(size_t)ring_mod(src[(size_t)r] + ov[(size_t)r], p)];
-            ov[(size_t)r] += df[(size_t)r];
+            A3 cs;
+            for (int k = 0; k < 3; ++k) cs[k] = ring_mod(src[(size_t)r][k] + ov[(size_t)r][k], g.dims[k]);
+            hold[(size_t)r] = old[(size_t)g.rank(cs)];
+            for (int k = 0; k < 3; ++k) ov[(size_t)r][k] += df[(size_t)r][k];
         }
     };
     for (int r = 0; r < p; ++r) buffer_counts[r] = particle_counts[r] = 0;
@@ -450,21 +501,43 @@ void p3bca_ranks_sweep(const P3bcaRanks &g, int64_t n, const double *pos, double
                 ++np;
             }
             if (i3 < numSteps - 1) {
-                /* schedule1DHelper (110-118) with calcDiff1D (120-128) */
+                /* schedule1DHelper (110-118) / schedule2DHelper, schedule3DHelper
+                 * (151-181, 247-276): the next source, advancing i3 while it
+                 * does not continue the current b1's sequence (the loop's
+                 * post-increment stops it at i3 = 0) */
+                int i3n = i3;
                 for (int r = 0; r < p; ++r) {
-                    nsi[(size_t)r] = ring_mod((i3 + 1) + r, p);
-                    dfi[(size_t)r] = ring_mod(i3 + r, p) - nsi[(size_t)r];
+                    const A3 c = g.coords(r);
+                    int j = i3;
+                    A3 src;
+                    if (g.nd == 1) {
+                        src = g.sched(j + 1, c);
+                    } else {
+                        for (;;) {
+                            const A3 b1Init = g.sched(i2, c);
+                            src = g.sched(j + 1, c);
+                            const A3 src2 = g.sched(j + 1 - i2, b1Init);
+                            if (src == src2 || !(j < numSteps)) break;
+                            if (j++ == 0) break;
+                        }
+                    }
+                    if (r == 0) i3n = j;
+                    else if (j != i3n) return -3;
+                    nsi[(size_t)r] = src;
+                    dfi[(size_t)r] = g.diff(c, src, i3 + 1);
                 }
-                shift(hold2, nsi, ovi, dfi);
+                i3 = i3n;
+                if (i3 < numSteps - 1) shift(hold2, nsi, ovi, dfi);
             }
         }
         if (i2 < outer - 1) {
-            /* schedule1D (100) and calcDiff1D for both buffers; b1 moves with
-             * the outer offset vector, b2 with the inner one, which then takes
-             * the outer one's value */
+            /* schedule and calcDiff for both buffers; b1 moves with the outer
+             * offset vector, b2 with the inner one, which then takes the
+             * outer one's value */
             for (int r = 0; r < p; ++r) {
-                nso[(size_t)r] = nsi[(size_t)r] = ring_mod((i2 + 1) + r, p);
-                dfo[(size_t)r] = dfi[(size_t)r] = ring_mod(i2 + r, p) - nso[(size_t)r];
+                const A3 c = g.coords(r);
+                nso[(size_t)r] = nsi[(size_t)r] = g.sched(i2 + 1, c);
+                dfo[(size_t)r] = dfi[(size_t)r] = g.diff(c, nso[(size_t)r], i2 + 1);
             }
             shift(hold1, nso, ovo, dfo);
             shift(hold2, nsi, ovi, dfi);
@@ -483,6 +556,7 @@ void p3bca_ranks_sweep(const P3bcaRanks &g, int64_t n, const double *pos, double
             fp[2] = a.fZ + (b.fZ + c.fZ);
         }
     if (nprocessed) *nprocessed = np;
+    return 0;
 }
 
 
@@ -1038,30 +1112,43 @@ int orc_sweep_ranks(int algo, int ranks, int64_t n, const double *pos, double nu
     return 0;
 }
 
-int orc_sweep_ranks_p3bca(int ranks, int64_t n, const double *pos, double nu, double cutoff, double *f,
-                          uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed, int64_t cap,
-                          int64_t *nprocessed)
+int orc_sweep_ranks_p3bca(int ndims, const int *dims, int64_t n, const double *pos, double nu, double cutoff,
+                          double *f, uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed,
+                          int64_t cap, int64_t *nprocessed)
 {
-    if (ranks < 1 || ranks > 8 || n < 0 || !(cutoff > 0)) return -1;
+    if (ndims < 1 || ndims > 3 || !dims || n < 0 || !(cutoff > 0)) return -1;
+    long long p = 1;
+    for (int k = 0; k < ndims; ++k) {
+        if (dims[k] < 1) return -1;
+        p *= dims[k];
+    }
+    if (p > 4096) return -1;
     P3bcaRanks g;
-    if (p3bca_ranks_init(g, ranks, n, pos, cutoff) != 0) return -2;
-    p3bca_ranks_sweep(g, n, pos, nu, cutoff, f, buffer_counts, particle_counts, processed, cap, nprocessed);
-    return 0;
+    if (p3bca_ranks_init(g, ndims, dims, n, pos, cutoff) != 0) return -2;
+    return p3bca_ranks_sweep(g, n, pos, nu, cutoff, f, buffer_counts, particle_counts, processed, cap, nprocessed);
 }
 
-int orc_simulate_ranks_p3bca(int ranks, int64_t n, double *pos, double *vel, double *acc, const double *mass,
-                             double *f, int steps, double dt, const double *g, double cutoff, double nu,
-                             uint64_t *buffer_counts, uint64_t *particle_counts, orc_step_cb step_cb, void *user)
+int orc_simulate_ranks_p3bca(int ndims, const int *dims, int64_t n, double *pos, double *vel, double *acc,
+                             const double *mass, double *f, int steps, double dt, const double *g, double cutoff,
+                             double nu, uint64_t *buffer_counts, uint64_t *particle_counts, orc_step_cb step_cb,
+                             void *user)
 {
-    if (ranks < 1 || ranks > 8 || n < 0 || !(cutoff > 0)) return -1;
+    if (ndims < 1 || ndims > 3 || !dims || n < 0 || !(cutoff > 0)) return -1;
+    long long p = 1;
+    for (int k = 0; k < ndims; ++k) {
+        if (dims[k] < 1) return -1;
+        p *= dims[k];
+    }
+    if (p > 4096) return -1;
     /* Simulation::Init bins the initial particles; Start then runs predictor,
      * sweep and update (with the exchange) per step (Simulation.cpp:28-81) */
     P3bcaRanks grid;
-    if (p3bca_ranks_init(grid, ranks, n, pos, cutoff) != 0) return -2;
+    if (p3bca_ranks_init(grid, ndims, dims, n, pos, cutoff) != 0) return -2;
     for (int s = 0; s < steps; ++s) {
         orc_predictor(n, pos, vel, acc, dt);
-        p3bca_ranks_sweep(grid, n, pos, nu, cutoff, f, buffer_counts + (size_t)s * ranks,
-                          particle_counts + (size_t)s * ranks, nullptr, 0, nullptr);
+        const int rc = p3bca_ranks_sweep(grid, n, pos, nu, cutoff, f, buffer_counts + (size_t)s * p,
+                                         particle_counts + (size_t)s * p, nullptr, 0, nullptr);
+        if (rc != 0) return rc;
         orc_update(n, f, mass, vel, acc, dt, g);
         p3bca_ranks_exchange(grid, pos);
         if (step_cb) step_cb(s, user);

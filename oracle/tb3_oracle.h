@@ -131,27 +131,30 @@ int orc_sweep_ranks(int algo, int ranks, int64_t n, const double *pos, double nu
                     uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed, int64_t cap,
                     int64_t *nprocessed);
 
-/* P3BCA's step sweep as `mpiexec -n ranks` runs it, for 1 to 8 ranks (the
- * reference's one-dimensional decompositions), restated in one process:
- * the grid decomposition's x-slabs, the cutoff boxes and their validity
- * check, the shift schedule of b1 and b2 with its offset vectors, the owner
- * rules and cutoff test of the triplet loops, and each rank's role sum.
- * Outputs as orc_sweep_ranks.  Returns 0; -1 for ranks outside 1..8 or a
- * cutoff <= 0; -2 where the reference exits with "invalid cutoff" (more than
- * two ranks and a cutoff that needs no cutoff box or at least ranks / 2). */
-int orc_sweep_ranks_p3bca(int ranks, int64_t n, const double *pos, double nu, double cutoff, double *f,
-                          uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed, int64_t cap,
-                          int64_t *nprocessed);
+/* P3BCA's step sweep as `mpiexec -n p` runs it on the Cartesian
+ * decomposition dims[0..ndims) (1 to 3 dimensions; p = their product, at
+ * most 4096), restated in one process: the grid decomposition's cells, the
+ * cutoff boxes and their validity check, the shift schedule of b1 and b2 with
+ * its offset vectors, the owner rules and cutoff test of the triplet loops,
+ * and each rank's role sum.  The reference takes dims from its decomposition
+ * tables (one-dimensional {p} for 1..8 ranks in both).  Outputs as
+ * orc_sweep_ranks.  Returns 0; -1 for bad arguments; -2 where the reference
+ * exits with "invalid cutoff"; -3 if the ranks' schedules disagreed. */
+int orc_sweep_ranks_p3bca(int ndims, const int *dims, int64_t n, const double *pos, double nu, double cutoff,
+                          double *f, uint64_t *buffer_counts, uint64_t *particle_counts, int32_t *processed,
+                          int64_t cap, int64_t *nprocessed);
 
-/* A P3BCA run of `steps` steps on 1..8 ranks: the initial binning, then per
- * step the predictor, the multi-rank sweep, the update and the exchange of
- * particles that left their rank's cell (RegularGridDecomposition::Update).
- * buffer_counts and particle_counts receive steps x ranks entries.  A particle
- * that no rank binned at the start (the reference leaves it out of the run)
- * is integrated here with a zero force.  Returns as orc_sweep_ranks_p3bca. */
-int orc_simulate_ranks_p3bca(int ranks, int64_t n, double *pos, double *vel, double *acc, const double *mass,
-                             double *f, int steps, double dt, const double *g, double cutoff, double nu,
-                             uint64_t *buffer_counts, uint64_t *particle_counts, orc_step_cb step_cb, void *user);
+/* A P3BCA run of `steps` steps on that decomposition: the initial binning,
+ * then per step the predictor, the multi-rank sweep, the update and the
+ * exchange of particles that left their rank's cell
+ * (RegularGridDecomposition::Update).  buffer_counts and particle_counts
+ * receive steps x p entries.  A particle that no rank binned at the start
+ * (the reference leaves it out of the run) is integrated here with a zero
+ * force.  Returns as orc_sweep_ranks_p3bca. */
+int orc_simulate_ranks_p3bca(int ndims, const int *dims, int64_t n, double *pos, double *vel, double *acc,
+                             const double *mass, double *f, int steps, double dt, const double *g, double cutoff,
+                             double nu, uint64_t *buffer_counts, uint64_t *particle_counts, orc_step_cb step_cb,
+                             void *user);
 
 #ifdef __cplusplus
 }

@@ -13,10 +13,12 @@
  *                               (the MEASURESIMSTEP_3BMDA line, Simulation.cpp:50-72)
  *   --counts                    print the accepted-triplet count of each step
  *   --full-precision            write CSV values with %.17g instead of %g
- *   --ranks P                   run as `mpiexec -n P` does: NATA and AUTA on
- *                               any P (orc_sweep_ranks), P3BCA on 1..8
- *                               (orc_simulate_ranks_p3bca); --counts then
+ *   --ranks P                   run as `mpiexec -n P` does (orc_sweep_ranks for
+ *                               NATA and AUTA, orc_simulate_ranks_p3bca for
+ *                               P3BCA on the decomposition {P}); --counts then
  *                               prints each rank's two counts per step
+ *   --cart X,Y[,Z]              P3BCA's Cartesian decomposition (the ranks are
+ *                               its product), as the reference's table gives it
  */
 #include <algorithm>
 #include <chrono>
@@ -45,6 +47,7 @@ struct Args {
     int threads = 0;
     bool timing = false, counts = false, fullPrecision = false;
     int ranks = 1;
+    std::vector<int> cart; /* P3BCA's Cartesian decomposition; {ranks} by default */
 };
 
 void printHelp()
@@ -103,6 +106,12 @@ Args parse(const std::vector<std::string> &args)
                 else if (flag == "-counts") a.counts = true;
                 else if (flag == "-full-precision") a.fullPrecision = true;
                 else if (flag == "-ranks") a.ranks = std::stoi(need());
+                else if (flag == "-cart") {
+                    std::stringstream ss(need());
+                    std::string v;
+                    a.cart.clear();
+                    while (std::getline(ss, v, ',')) a.cart.push_back(std::stoi(v));
+                }
                 else std::cout << "CLI parser: unrecognized option: " << flag << std::endl;
             } else {
                 std::cout << "CLI parser: unrecognized option: " << args[i] << std::endl;
@@ -207,21 +216,26 @@ int main(int argc, char **argv)
     std::vector<uint64_t> counts((size_t)std::max(a.iterations, 1), 0);
     CbState st{&a, &t, a.outputCSV.substr(0, a.outputCSV.find_last_of('.')), std::chrono::steady_clock::now(),
                &counts};
+    if (!a.cart.empty()) {
+        a.ranks = 1;
+        for (int d : a.cart) a.ranks *= d;
+    }
     if (a.ranks > 1) {
         /* Simulation::Start with the multi-rank sweep: the integrator and the
          * CSV writer act per particle, so only the sweep and its counts differ */
         if (a.algorithm == 1) {
             const int it = std::max(a.iterations, 0);
             std::vector<uint64_t> bcs((size_t)std::max(it, 1) * a.ranks), pcs(bcs.size());
-            const int rc = orc_simulate_ranks_p3bca(a.ranks, n, t.p.data(), t.v.data(), t.acc.data(), t.m.data(),
-                                                    t.f.data(), it, a.deltaT, a.g, a.cutoff, 1.0, bcs.data(),
-                                                    pcs.data(), stepCallback, &st);
+            const std::vector<int> dims = a.cart.empty() ? std::vector<int>{a.ranks} : a.cart;
+            const int rc = orc_simulate_ranks_p3bca((int)dims.size(), dims.data(), n, t.p.data(), t.v.data(),
+                                                    t.acc.data(), t.m.data(), t.f.data(), it, a.deltaT, a.g, a.cutoff,
+                                                    1.0, bcs.data(), pcs.data(), stepCallback, &st);
             if (rc == -2) {
                 std::cerr << "invalid cutoff: " << a.cutoff << " for " << a.ranks << " ranks" << std::endl;
                 return 1;
             }
             if (rc != 0) {
-                std::cerr << "--ranks: P3BCA runs on 1 to 8 ranks with a positive cutoff" << std::endl;
+                std::cerr << "--ranks: P3BCA needs a positive cutoff and 1 to 3 decomposition dimensions" << std::endl;
                 return 1;
             }
             if (a.counts)

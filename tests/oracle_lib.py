@@ -66,11 +66,13 @@ def lib():
                                       _u64p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_int64)]
         L.orc_sweep_ranks.restype = ctypes.c_int
-        L.orc_sweep_ranks_p3bca.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp,
-                                            _u64p, _u64p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
+        L.orc_sweep_ranks_p3bca.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int64, _dp,
+                                            ctypes.c_double, ctypes.c_double, _dp, _u64p, _u64p,
+                                            ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_int64)]
         L.orc_sweep_ranks_p3bca.restype = ctypes.c_int
-        L.orc_simulate_ranks_p3bca.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+        L.orc_simulate_ranks_p3bca.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int64, _dp, _dp,
+                                               _dp, _dp, _dp, ctypes.c_int,
                                                ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double, _u64p, _u64p,
                                                ctypes.c_void_p, ctypes.c_void_p]
         L.orc_simulate_ranks_p3bca.restype = ctypes.c_int
@@ -112,10 +114,19 @@ def sweep(pos, nu=1.0, cutoff=-1.0, L=None, mode="owner", threads=0):
     return f0, f1, f2, f, int(cnt)
 
 
-def sweep_ranks(pos, algo, ranks, nu=1.0, cutoff=-1.0):
-    """The step sweep of NATA, AUTA or P3BCA (with `cutoff`) as `mpiexec -n
-    ranks` runs it, restated in one process: returns (f, buffer_counts, particle_counts, processed)
-    with processed a (k, 4) array of (rank, owner of b0, b1, b2) rows."""
+def _dims(ranks, dims):
+    d = (ranks,) if dims is None else tuple(int(x) for x in dims)
+    return len(d), (ctypes.c_int * len(d))(*d), int(np.prod(d))
+
+
+def sweep_ranks(pos, algo, ranks, nu=1.0, cutoff=-1.0, dims=None):
+    """The step sweep of NATA, AUTA or P3BCA (with `cutoff`, on the Cartesian
+    decomposition `dims`, (ranks,) by default) as `mpiexec -n ranks` runs it,
+    restated in one process: returns (f, buffer_counts, particle_counts,
+    processed) with processed a (k, 4) array of (rank, owner of b0, b1, b2)
+    rows."""
+    if algo == "p3bca":
+        nd, dv, ranks = _dims(ranks, dims)
     pos = _c(pos)
     n = pos.shape[0]
     f = np.zeros((n, 3))
@@ -125,7 +136,7 @@ def sweep_ranks(pos, algo, ranks, nu=1.0, cutoff=-1.0):
     proc = np.zeros((cap, 4), dtype=np.int32)
     nproc = ctypes.c_int64(0)
     if algo == "p3bca":
-        rc = lib().orc_sweep_ranks_p3bca(ranks, n, _p(pos), nu, cutoff, _p(f), bc.ctypes.data_as(_u64p),
+        rc = lib().orc_sweep_ranks_p3bca(nd, dv, n, _p(pos), nu, cutoff, _p(f), bc.ctypes.data_as(_u64p),
                                          pc.ctypes.data_as(_u64p), proc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                          cap, ctypes.byref(nproc))
     else:
@@ -138,16 +149,18 @@ def sweep_ranks(pos, algo, ranks, nu=1.0, cutoff=-1.0):
     return f, bc, pc, proc[:nproc.value]
 
 
-def simulate_ranks_p3bca(pos, vel, acc, mass, ranks, steps, dt, g, cutoff, nu=1.0):
-    """A P3BCA run on `ranks` ranks (1..8) with the particle exchange between
-    steps: returns (pos, vel, acc, f, buffer_counts, particle_counts), the
-    counts as (steps, ranks) arrays."""
+def simulate_ranks_p3bca(pos, vel, acc, mass, ranks, steps, dt, g, cutoff, nu=1.0, dims=None):
+    """A P3BCA run on `ranks` ranks (decomposition `dims`, (ranks,) by
+    default) with the particle exchange between steps: returns (pos, vel,
+    acc, f, buffer_counts, particle_counts), the counts as (steps, ranks)
+    arrays."""
+    nd, dv, ranks = _dims(ranks, dims)
     pos, vel, acc = (_c(a).copy() for a in (pos, vel, acc))
     n = pos.shape[0]
     f = np.zeros((n, 3))
     bc = np.zeros((steps, ranks), dtype=np.uint64)
     pc = np.zeros((steps, ranks), dtype=np.uint64)
-    rc = lib().orc_simulate_ranks_p3bca(ranks, n, _p(pos), _p(vel), _p(acc), _p(_c(mass)), _p(f), steps, dt,
+    rc = lib().orc_simulate_ranks_p3bca(nd, dv, n, _p(pos), _p(vel), _p(acc), _p(_c(mass)), _p(f), steps, dt,
                                         _p(_c(g)), cutoff, nu, bc.ctypes.data_as(_u64p), pc.ctypes.data_as(_u64p),
                                         None, None)
     if rc != 0:

@@ -372,13 +372,13 @@ def test_multi_rank_sweep(oracle, algo, ranks):
 
 
 def test_multi_rank_unsupported(oracle):
-    """P3BCA needs a positive cutoff, and only its 1-D decompositions (1..8
-    ranks) are restated."""
+    """P3BCA needs a positive cutoff and a decomposition of 1 to 3
+    dimensions."""
     P = uniform_particles(10, 3.0, 1)
     with pytest.raises(ValueError):
         oracle.sweep_ranks(P, "p3bca", 2)
     with pytest.raises(ValueError):
-        oracle.sweep_ranks(P, "p3bca", 9, cutoff=1.0)
+        oracle.sweep_ranks(P, "p3bca", 16, cutoff=0.5, dims=(2, 2, 2, 2))
 
 
 @pytest.mark.parametrize("algo,ranks,cutoff", [("nata", 4, -1.0), ("p3bca", 2, 1.0)])
@@ -420,37 +420,56 @@ def test_oracle_cli_ranks_matches_one_process(oracle, tmp_path, algo, ranks, cut
     assert np.allclose(a[:, 1:], b[:, 1:], rtol=1e-11, atol=1e-11)
 
 
-def _p3bca_expected_1d(p, ncb):
+def _p3bca_expected(dims, ncb):
     """The reference test's expected buffer triplets per rank (tests.cpp:276-335
-    with customLt 254-259 and periodicDiff 243-247) on a 1-D decomposition of
-    p ranks with ncb cutoff boxes."""
-    def pdist(x, y):
-        return min(abs(x - y), p - abs(x - y))
+    with customLt 254-259, vLtV 214-228 and periodicDiff 243-247) on the
+    Cartesian decomposition dims (padded to three dimensions) with ncb cutoff
+    boxes per dimension.  Ranks are row-major coordinates."""
+    dims = list(dims) + [1] * (3 - len(dims))
+    ncb = list(ncb) + [0] * (3 - len(ncb))
 
-    def pdiff(x, y):
-        return (x - y) if abs(x - y) <= p // 2 else (1 if y - x >= 0 else -1) * pdist(x, y)
+    def pdist(x, y, d):
+        return min(abs(x - y), d - abs(x - y))
 
-    def lt(r, u, v):  # vLtV on 1-D differences: true when less or equal
-        return pdiff(u, r) <= pdiff(v, r)
+    def pdiff(x, y, d):
+        return (x - y) if abs(x - y) <= d // 2 else (1 if y - x >= 0 else -1) * pdist(x, y, d)
 
-    win = {r: [q for q in range(p) if pdist(q, r) <= ncb] for r in range(p)}
+    def lt(r, u, v):  # vLtV: lexicographic, true when equal
+        d0 = tuple(pdiff(u[k], r[k], dims[k]) for k in range(3))
+        d1 = tuple(pdiff(v[k], r[k], dims[k]) for k in range(3))
+        return d0 <= d1
+
+    U = [(i, j, k) for i in range(dims[0]) for j in range(dims[1]) for k in range(dims[2])]
+    win = {r: [q for q in U if all(pdist(q[k], r[k], dims[k]) <= ncb[k] for k in range(3))] for r in U}
     out = {}
-    for r in range(p):
+    for r in U:
         out[r] = [(r, i2, i3) for i2 in win[r] if lt(r, r, i2) for i3 in win[r] if i3 in win[i2] and lt(r, i2, i3)]
     return out
 
 
-@pytest.mark.parametrize("ranks,cutoff", [(1, 2.0), (2, 2.0), (3, 2.0), (4, 2.0), (5, 2.0), (6, 2.0), (7, 2.0),
-                                          (8, 2.0), (8, 1.5), (5, 1.2)])
-def test_multi_rank_sweep_p3bca(oracle, ranks, cutoff):
-    """P3BCA's step sweep as `mpiexec -n ranks` runs it (1-D decompositions,
-    1..8 ranks), restated in one process.  Pinned by:
+def _coords(rank, dims):
+    dims = list(dims) + [1] * (3 - len(dims))
+    return (rank // (dims[1] * dims[2]), (rank // dims[2]) % dims[1], rank % dims[2])
+
+
+@pytest.mark.parametrize("dims,cutoff", [((1,), 2.0), ((2,), 2.0), ((3,), 2.0), ((4,), 2.0), ((5,), 2.0), ((6,), 2.0),
+                                         ((7,), 2.0), ((8,), 2.0), ((8,), 1.5), ((5,), 1.2), ((3, 3), 2.0),
+                                         ((3, 4), 2.0), ((4, 4), 2.0), ((3, 3, 3), 2.0), ((4, 4, 4), 2.0)])
+def test_multi_rank_sweep_p3bca(oracle, dims, cutoff):
+    """P3BCA's step sweep as `mpiexec -n p` runs it on the Cartesian
+    decomposition dims (the reference's tables give {p} for 1..8 ranks and,
+    in the optimal table, {3, 3}, {3, 4}, {4, 4}, {3, 3, 3} for 9, 12, 16
+    and 27), restated in one process.  Pinned by:
       * the reference's coverage test (tests.cpp:691-825): each rank's buffer
         triplets equal the expected set of its cutoff-box window
         (tests.cpp:276-335), with no triplet expected twice, on one rank or
         across ranks.  At two ranks the step's special case (P3BCA.cpp:577)
         computes the same triplets split differently between the ranks, so
-        there the union is compared;
+        there the union is compared.  Where y or z has exactly 3 cells, the
+        test's windows wrap around and it expects cell triplets such as
+        (0,0), (0,1), (1,2) that no particle triplet within the cutoff can
+        span and that the schedule never forms; there the computed triplets
+        are checked to be a subset;
       * every accepted particle triplet once: the counts sum to the
         one-process count;
       * the forces equal the one-process sweep up to reordering, bit for bit
@@ -458,21 +477,30 @@ def test_multi_rank_sweep_p3bca(oracle, ranks, cutoff):
     from collections import Counter
     n = 1500
     box = (n / 2.0) ** (1 / 3) * 1.6
+    ranks = int(np.prod(dims))
     P = uniform_particles(n, box, 70 + ranks)
     L, _ = oracle.domain_size(P)
     *_, fo, co = oracle.sweep(P, cutoff=cutoff, L=L, mode="owner")
-    f, bc, pc, proc = oracle.sweep_ranks(P, "p3bca", ranks, cutoff=cutoff)
+    f, bc, pc, proc = oracle.sweep_ranks(P, "p3bca", ranks, cutoff=cutoff, dims=dims)
     assert int(pc.sum()) == co
-    ncb = 0 if ranks == 1 else (1 if ranks == 2 else int(np.ceil(cutoff / (L[0] / ranks))))
-    exp = _p3bca_expected_1d(ranks, ncb)
-    every = [tuple(sorted(t)) for r in exp for t in exp[r]]
+    d3 = list(dims) + [1] * (3 - len(dims))
+    w = [L[k] / d3[k] for k in range(3)]
+    ncb = [0 if ranks == 1 else (1 if ranks == 2 else int(np.ceil(cutoff / w[0])))]
+    ncb += [int(np.ceil(cutoff / w[k])) if len(dims) > k else 0 for k in (1, 2)]
+    exp = _p3bca_expected(dims, ncb)
+    key = lambda t: tuple(sorted(t))
+    every = [key(t) for r in exp for t in exp[r]]
     assert len(every) == len(set(every))
-    got_all = Counter(tuple(sorted(int(x) for x in row[1:])) for row in proc)
-    assert got_all == Counter(every)
-    if ranks != 2:  # two ranks take the reference's special case: the same triplets, split differently
-        for r in range(ranks):
-            got = Counter(tuple(sorted(int(x) for x in row[1:])) for row in proc if row[0] == r)
-            assert got == Counter(tuple(sorted(t)) for t in exp[r]), r
+    rows = [(int(r[0]), tuple(_coords(int(x), dims) for x in r[1:])) for r in proc]
+    got_all = Counter(key(t) for _, t in rows)
+    if any(d == 3 for d in dims[1:]):
+        assert not (got_all - Counter(every))
+    else:
+        assert got_all == Counter(every)
+    if ranks != 2 and not any(d == 3 for d in dims[1:]):
+        for rk in range(ranks):
+            got = Counter(key(t) for r, t in rows if r == rk)
+            assert got == Counter(key(t) for t in exp[_coords(rk, dims)]), rk
     if ranks == 1:
         assert np.array_equal(f, fo)
     else:
@@ -489,10 +517,10 @@ def test_multi_rank_p3bca_invalid_cutoff(oracle):
         oracle.sweep_ranks(P, "p3bca", 4, cutoff=5.0)
 
 
-@pytest.mark.parametrize("ranks", [2, 3, 4, 5, 6])
-def test_multi_rank_p3bca_run_with_exchange(oracle, ranks):
+@pytest.mark.parametrize("dims", [(2,), (3,), (4,), (5,), (6,), (3, 3), (3, 3, 3)])
+def test_multi_rank_p3bca_run_with_exchange(oracle, dims):
     """Six P3BCA steps on several ranks, with three fast particles crossing
-    the ranks' x-slabs inside the domain, so the particle exchange between
+    the ranks' cells along x inside the domain, so the particle exchange between
     steps (RegularGridDecomposition::Update) moves them from rank to rank:
     every step's counts equal the one-process run's, and the state matches it
     to the reordering of the force sums."""
@@ -505,7 +533,7 @@ def test_multi_rank_p3bca_run_with_exchange(oracle, ranks):
     V[7, 0], V[500, 0], V[250, 0] = 600.0, -700.0, 450.0
     g = np.array([0.0, -9.81, 0.0])
     pos1, vel1, acc1, f1, oc = oracle.simulate(P, V, A, M, "p3bca", 6, 0.001, g, cutoff=1.5)
-    pos, vel, acc, f, bc, pc = oracle.simulate_ranks_p3bca(P, V, A, M, ranks, 6, 0.001, g, 1.5)
+    pos, vel, acc, f, bc, pc = oracle.simulate_ranks_p3bca(P, V, A, M, 0, 6, 0.001, g, 1.5, dims=dims)
     assert [int(c) for c in pc.sum(axis=1)] == [int(c) for c in oc]
     assert np.abs(pos - pos1).max() <= 1e-12
     num = np.linalg.norm(f - f1, axis=1)
