@@ -502,14 +502,15 @@ __device__ void order_runs(int ncell, const Start &start, const unsigned *__rest
         __syncthreads();
         for (int c = lo + t; c < hi; c += 256) atomicAdd(&hist[bin(c)], 1);
         __syncthreads();
-        if (t == 0) {
-            int acc = 0;
-            for (int b = 0; b < 256; ++b) {
-                const int h = hist[b];
-                hist[b] = acc;
-                acc += h;
-            }
+        // exclusive scan of the 256 bins, one per thread (Hillis-Steele)
+        const int own = hist[t];
+        for (int o = 1; o < 256; o <<= 1) {
+            const int u = t >= o ? hist[t - o] : 0;
+            __syncthreads();
+            hist[t] += u;
+            __syncthreads();
         }
+        hist[t] -= own;
         __syncthreads();
         for (int c = lo + t; c < hi; c += 256) order[lo + atomicAdd(&hist[bin(c)], 1)] = c;
         __syncthreads();
