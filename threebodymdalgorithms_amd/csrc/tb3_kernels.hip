@@ -482,11 +482,11 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
 // is the triplet count of its last sweep (most particles move little in a
 // step), or 16 n^2 for n particles before it has one.  A counting sort on
 // 1/16-octave bins of the work (256 bins; ties in any order), one workgroup
-// per run (workgroups 0..7, or fewer, looping).  start(c) is cell c's first
-// slot.
+// per run: the calling workgroup sorts runs r0, r0 + rstep, ... below 8.
+// start(c) is cell c's first slot.
 template <class Start>
 __device__ void order_runs(int ncell, const Start &start, const unsigned *__restrict__ cell_work,
-                           int *__restrict__ order, int *hist)
+                           int *__restrict__ order, int *hist, int r0, int rstep)
 {
     const int t = threadIdx.x;
     const int run = (ncell + 7) >> 3;
@@ -496,7 +496,7 @@ __device__ void order_runs(int ncell, const Start &start, const unsigned *__rest
         const float k = w ? (float)w : 16.0f * (float)n * (float)n;
         return 255 - min(255, (int)(16.0f * __log2f(1.0f + k)));
     };
-    for (int r = blockIdx.x; r < 8; r += gridDim.x) {
+    for (int r = r0; r < 8; r += rstep) {
         const int lo = r * run, hi = min(lo + run, ncell);
         hist[t] = 0;
         __syncthreads();
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
                                                       int *__restrict__ order, const unsigned *__restrict__ cell_work)
 {
     __shared__ int hist[256];
-    order_runs(ncell, [&](int c) { return cell_start[c]; }, cell_work, order, hist);
+    order_runs(ncell, [&](int c) { return cell_start[c]; }, cell_work, order, hist, blockIdx.x, gridDim.x);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int c = cell_of[p];
@@ -546,7 +546,11 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
 // rank in its cell, so the slot is start + rank and the counters are only
 // read here; the last workgroup to finish (a ticket that wraps back to 0)
 // clears them for the next step, after every workgroup has read them.
-// Workgroup 0 publishes the starts for the sweep kernels.
+// Workgroup 0 publishes the starts for the sweep kernels.  The grid is the
+// nsc = ceil(n / 256) scatter workgroups and 8 more that each order one
+// XCD's run (order_runs), so the ordering runs beside the scatter instead of
+// after it.  Each scatter thread loads its particle before the scan, which
+// overlaps the two memory round trips.
 constexpr int FUSED_SCAN_CELLS = 8192;
 
 __global__ __launch_bounds__(256) void k_cell_scatter_fused(
@@ -554,13 +558,23 @@ __global__ __launch_bounds__(256) void k_cell_scatter_fused(
     const int *__restrict__ cell_of, const int *__restrict__ cell_rank, int *__restrict__ cell_cnt,
     int *__restrict__ cell_start, int *__restrict__ cell_idx, double *__restrict__ xs, double *__restrict__ ys,
     double *__restrict__ zs, int ncell, int *__restrict__ order, const unsigned *__restrict__ cell_work,
-    unsigned *__restrict__ ticket)
+    unsigned *__restrict__ ticket, int nsc)
 {
     __shared__ int lst[FUSED_SCAN_CELLS + 1];
     __shared__ int part[256];
     __shared__ int hist[256];
     __shared__ int s_last;
     const int t = threadIdx.x;
+    const int p = blockIdx.x * blockDim.x + t;  // >= n in the ordering workgroups
+    int pcell = 0, prank = 0;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (p < n) {
+        pcell = cell_of[p];
+        prank = cell_rank[p];
+        px = x[p];
+        py = y[p];
+        pz = z[p];
+    }
     // exclusive scan of the counts: coalesced load, a serial scan of each
     // thread's chunk, then a scan of the 256 chunk sums
     for (int c = t; c < ncell; c += 256) lst[c] = cell_cnt[c];
@@ -588,15 +602,14 @@ __global__ __launch_bounds__(256) void k_cell_scatter_fused(
     __syncthreads();
     if (blockIdx.x == 0)
         for (int c = t; c <= ncell; c += 256) cell_start[c] = lst[c];
-    order_runs(ncell, [&](int c) { return lst[c]; }, cell_work, order, hist);
+    if (blockIdx.x >= nsc) order_runs(ncell, [&](int c) { return lst[c]; }, cell_work, order, hist, blockIdx.x - nsc, 8);
 
-    const int p = blockIdx.x * blockDim.x + t;
     if (p < n) {
-        const int slot = lst[cell_of[p]] + cell_rank[p];
+        const int slot = lst[pcell] + prank;
         cell_idx[slot] = p;
-        xs[slot] = x[p];
-        ys[slot] = y[p];
-        zs[slot] = z[p];
+        xs[slot] = px;
+        ys[slot] = py;
+        zs[slot] = pz;
     }
     __syncthreads();
     if (t == 0) s_last = atomicInc(ticket, gridDim.x - 1) == gridDim.x - 1;
@@ -2214,9 +2227,10 @@ void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double
     if (n > 0 && g.ncell <= FUSED_SCAN_CELLS) {
         hipLaunchKernelGGL(k_bin<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, predict, dt, h, zero_f,
                            grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank, st);
-        hipLaunchKernelGGL(k_cell_scatter_fused, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d.x, d.y, d.z,
-                           g.cell_of, g.cell_rank, g.cell_cnt, g.cell_start, g.cell_idx, g.xs, g.ys, g.zs, g.ncell,
-                           g.order, g.cell_work, g.scatter_ticket);
+        const int nsc = blocks_for(n, 256);
+        hipLaunchKernelGGL(k_cell_scatter_fused, dim3(nsc + 8), dim3(256), 0, s, n, d.x, d.y, d.z, g.cell_of,
+                           g.cell_rank, g.cell_cnt, g.cell_start, g.cell_idx, g.xs, g.ys, g.zs, g.ncell, g.order,
+                           g.cell_work, g.scatter_ticket, nsc);
         return;
     }
     if (n > 0)
