@@ -207,12 +207,17 @@ __global__ __launch_bounds__(NSLOT) void k_finalize(Status *st) { fold_slots(st)
 
 // Particle::UpdatePredictorStage (utility/structs.hpp:40-55):
 // p = (p + v*dt) + aX*h with h = (dt*dt)*0.5, aX for all three axes.
+__device__ __forceinline__ double predicted(double x, double v, double aX, double dt, double h)
+{
+    return x + v * dt + aX * h;
+}
+
 __device__ __forceinline__ void predict_one(const DeviceState &d, int p, double dt, double h)
 {
     const double aX = d.ax[p];
-    d.x[p] = d.x[p] + d.vx[p] * dt + aX * h;
-    d.y[p] = d.y[p] + d.vy[p] * dt + aX * h;
-    d.z[p] = d.z[p] + d.vz[p] * dt + aX * h;
+    d.x[p] = predicted(d.x[p], d.vx[p], aX, dt, h);
+    d.y[p] = predicted(d.y[p], d.vy[p], aX, dt, h);
+    d.z[p] = predicted(d.z[p], d.vz[p], aX, dt, h);
 }
 
 __global__ void k_predict(DeviceState d, double dt, double h)
@@ -458,13 +463,22 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.n) return;
-    if (predict && !st->halt) predict_one(d, p, dt, h);
+    // the predicted position stays in registers for the binning below
+    double px = d.x[p], py = d.y[p], pz = d.z[p];
+    if (predict && !st->halt) {
+        const double aX = d.ax[p];
+        px = predicted(px, d.vx[p], aX, dt, h);
+        py = predicted(py, d.vy[p], aX, dt, h);
+        pz = predicted(pz, d.vz[p], aX, dt, h);
+        d.x[p] = px;
+        d.y[p] = py;
+        d.z[p] = pz;
+    }
     if (zero_f) {
         d.fx[p] = 0.0;
         d.fy[p] = 0.0;
         d.fz[p] = 0.0;
     }
-    const double px = d.x[p], py = d.y[p], pz = d.z[p];
     if (!finite3(px, py, pz)) atomicOr(&st->flags[0], 1);
     const SysGrid sg = grid_of_particle(g, p);
     int cx, cy, cz;
