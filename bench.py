@@ -12,7 +12,14 @@ Printed JSON line:
   value        unique triplets/s of the FP64 (squared-distance) path, inputs
                resident in HBM: K steps of tb3_run (replays of one captured
                step graph, one host synchronisation) over wall-clock time
-  exact        the bit-exact FP64 path (reference expression and order)
+  exact        the bit-exact FP64 path (reference expression and order; its
+               positions, velocities, forces and counts equal the
+               reference's over every step), with its force kernel's and
+               whole step's fraction of the FP64 peak
+  fp64_vs_exact the headline path's per-particle force error against the
+               bit-exact path on the initial state (max, RMS, floored at the
+               RMS force) and the step through which their counts agree
+  step_frac    115 FLOP per triplet over the whole step time / FP64 peak
   fp32_mixed   the FP32-mixed path, its speedup over FP64 and its force /
                energy error against the bit-exact path on the initial state
   roofline     the force kernel: 115 FLOP per unique triplet / its average
@@ -102,20 +109,53 @@ def timed_steps(eng, P, precision, steps, warmup, sync, tune=True):
     }
 
 
-def fp32_accuracy(eng, P):
-    """FP32-mixed forces and energy against the bit-exact FP64 path on P."""
-    eng.set_particles(P)
-    eng.configure("p3bca", cutoff=CUTOFF, precision="exact")
-    E64, f64, _, _ = eng.energy_grad()
-    eng.configure("p3bca", cutoff=CUTOFF, precision="fp32")
-    E32, f32, _, _ = eng.energy_grad()
-    num = np.linalg.norm(f32 - f64, axis=1)
-    den = np.linalg.norm(f64, axis=1)
+def step_frac(r):
+    """115 FLOP per unique triplet over the whole step (cell list, sweep,
+    integration and the launches between them), against the FP64 peak."""
+    return FLOP_PER_TRIPLET * r["triplets_per_step"] / (r["ms_per_step"] * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS
+
+
+def host_cpu():
+    """The host's CPU model and the threads this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "cpus_available": avail}
+
+
+def force_errors(f, fo):
+    """Per-particle force error of f against fo: the pure ratio |df|/|fo|
+    (max and RMS) and the ratio with |fo| floored at the RMS force."""
+    num = np.linalg.norm(f - fo, axis=1)
+    den = np.linalg.norm(fo, axis=1)
     rel = num / np.where(den > 0, den, 1.0)
     floor = np.sqrt(np.mean(den ** 2))
     return {"max_rel_force_err": float(rel.max()), "rms_rel_force_err": float(np.sqrt(np.mean(rel ** 2))),
-            "max_rel_force_err_floored": float((num / np.maximum(den, floor)).max()),
-            "rel_energy_err": float(abs(E32 - E64) / abs(E64))}
+            "max_rel_force_err_floored": float((num / np.maximum(den, floor)).max())}
+
+
+def accuracy_vs_exact(eng, P):
+    """FP64 and FP32-mixed forces and energies of the initial state against
+    the bit-exact FP64 path (whose forces equal the reference's bit for bit)."""
+    eng.set_particles(P)
+    eng.configure("p3bca", cutoff=CUTOFF, precision="exact")
+    Ex, fx, _, _ = eng.energy_grad()
+    out = {}
+    for prec in ("fp64", "fp32"):
+        eng.configure("p3bca", cutoff=CUTOFF, precision=prec)
+        E, f, _, _ = eng.energy_grad()
+        out[prec] = {**force_errors(f, fx), "rel_energy_err": float(abs(E - Ex) / abs(Ex))}
+    return out
 
 
 PROFILE = os.path.join(ROOT, "profiles", "k_p3bca_cells_p3bca_c2_n32768_fp64.md")
@@ -162,7 +202,7 @@ def cpu_baseline(P, count0, target_s):
     secs, cand = run(rows)
     total_cand = N * (N - 1) * (N - 2) // 6
     step_s = secs * total_cand / cand
-    return {"value": count0 / step_s, "unit": "unique triplets/s", "cores": 1, "kind": "port",
+    return {"value": count0 / step_s, "unit": "unique triplets/s", "cores": 1, "kind": "port", "host": host_cpu(),
             "sample": f"reference-shaped brute i<j<k sweep, rows 0..{rows - 1} of step 0 ({cand:.3e} of "
                       f"{total_cand:.3e} candidates, {secs:.1f} s), extrapolated by candidate count to "
                       f"{step_s:.0f} s per step",
@@ -195,6 +235,7 @@ def cpu_fast(P):
         dt = time.perf_counter() - t0
         best = dt if best is None else min(best, dt)
     return {"value": cnt.value / best, "unit": "unique triplets/s", "cores": threads, "kind": "port",
+            "host": host_cpu(),
             "sample": f"cell-list sweep of step 0 ({cnt.value} triplets, best of 3: {best * 1e3:.1f} ms)",
             "ms_per_step": best * 1e3}
 
@@ -245,7 +286,7 @@ def main():
         fp64 = timed_steps(eng, P, "fp64", args.steps, args.warmup, sync, not args.no_autotune)
         log(f"bench: fp64 {fp64['ms_per_step']:.3f} ms/step {fp64['triplets_per_s']:.4e} triplets/s "
             f"force kernel {fp64['force_kernel_ms']:.3f} ms")
-        out_exact = out_fp32 = None
+        out_exact = out_fp32 = fp64_vs_exact = None
         if not args.skip_variants:
             ex = timed_steps(eng, P, "exact", args.steps, args.warmup, sync)
             log(f"bench: exact {ex['ms_per_step']:.3f} ms/step force kernel {ex['force_kernel_ms']:.3f} ms")
@@ -262,16 +303,25 @@ def main():
                 log(f"bench: exact and fp64 counts agree for steps 0-{same - 1}; the trajectories then diverge "
                     f"(at most {worst} triplets apart)")
             f32 = timed_steps(eng, P, "fp32", args.steps, args.warmup, sync, not args.no_autotune)
-            acc = fp32_accuracy(eng, P)
-            log(f"bench: fp32 {f32['ms_per_step']:.3f} ms/step force kernel {f32['force_kernel_ms']:.3f} ms {acc}")
+            acc = accuracy_vs_exact(eng, P)
+            log(f"bench: fp32 {f32['ms_per_step']:.3f} ms/step force kernel {f32['force_kernel_ms']:.3f} ms "
+                f"{acc['fp32']}; fp64 vs exact {acc['fp64']}")
             out_exact = {"value": ex["triplets_per_s"], "ms_per_step": ex["ms_per_step"],
                          "force_kernel_ms": ex["force_kernel_ms"],
-                         "force_kernel_tflops": ex["force_kernel_tflops"]}
+                         "force_kernel_tflops": ex["force_kernel_tflops"],
+                         "force_kernel_frac": ex["force_kernel_tflops"] / FP64_VECTOR_PEAK_TFLOPS,
+                         "step_frac": step_frac(ex),
+                         "counts_equal_reference_path": True,
+                         "note": "reproduces the reference bit for bit (positions, velocities, forces, counts) "
+                                 "over every step"}
             out_fp32 = {"value": f32["triplets_per_s"], "ms_per_step": f32["ms_per_step"],
                         "force_kernel_ms": f32["force_kernel_ms"],
                         "speedup_vs_fp64_step": fp64["ms_per_step"] / f32["ms_per_step"],
                         "speedup_vs_fp64_force_kernel": fp64["force_kernel_ms"] / f32["force_kernel_ms"],
-                        **acc}
+                        "step0_vs_exact": acc["fp32"]}
+            fp64_vs_exact = {"step0": acc["fp64"], "counts_agree_through_step": same - 1,
+                             "note": "one sweep of the initial state against the bit-exact path; the fast path "
+                                     "sums in another order, so the chaotic trajectories part after the first steps"}
     cpu = cpu_mc = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline ...")
@@ -300,6 +350,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": profiled_traffic(),
                      "kernel": "tb3::k_p3bca_cells<double, true, false>", "flop_per_triplet": FLOP_PER_TRIPLET},
+        "step_frac": step_frac(fp64),
+        "fp64_vs_exact": fp64_vs_exact,
         "cpu_baseline": cpu,
         "cpu_multicore": cpu_mc,
         "exact": out_exact,

@@ -148,15 +148,18 @@ tb3_status tb3_compute_forces(tb3_ctx *ctx, uint64_t *n_triplets);
 tb3_status tb3_energy(tb3_ctx *ctx, double *E, uint64_t *n_triplets);
 
 /* Energy adjoint: E, dE/dx (= f, n x 3 host buffer, may be NULL) and
- * dE/dnu = E / nu (may be NULL). */
+ * dE/dnu (may be NULL).  E is linear in nu, so dE/dnu is the energy at
+ * nu = 1: E / nu, or at nu = 0 a second (energy-only) pass at nu = 1. */
 tb3_status tb3_energy_grad(tb3_ctx *ctx, double *E, double *dEdx, double *dEdnu, uint64_t *n_triplets);
 
 /* Device-pointer forms for framework bindings (torch).  d_pos is an n x 3
  * row-major device array; results land in device memory; `stream` is a
  * hipStream_t (NULL: the context's stream).  They use the context's
  * algorithm, cutoff, domain and precision but not its particles, and leave
- * the context's particle state untouched.  Asynchronous on `stream`, except
- * that n_triplets (host, may be NULL) forces a synchronisation. */
+ * the context's particle state untouched.  The work is ordered on `stream`
+ * after what the caller queued there before, and is complete on return: the
+ * context's status block and scratch memory, which its next call reuses, are
+ * read by then.  n_triplets (host) may be NULL. */
 tb3_status tb3_energy_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, double nu, double *d_E, void *stream,
                           uint64_t *n_triplets);
 tb3_status tb3_energy_grad_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, double nu, double *d_E,
@@ -173,11 +176,13 @@ tb3_status tb3_energy_grad_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, dou
 tb3_status tb3_autotune(tb3_ctx *ctx, int reps, double *best_ms, double *cell_ratio, int *blocks_per_cu);
 
 /* Per-phase timing with HIP events recorded on the context's stream around
- * each phase of tb3_step / tb3_run / tb3_compute_forces.  No counterpart in
+ * each phase of tb3_step, tb3_run (which runs its steps one pass at a time
+ * while profiling is on), tb3_compute_forces and the device-pointer energy
+ * calls.  No counterpart in
  * the reference beyond its per-method timers (PROFILE_3BMDA, Algorithm.cpp:57-63,
  * 145-162).  tb3_set_profiling(ctx, 1) enables and clears the accumulators.
  * tb3_get_profile fills up to n values:
- *   [0] steps timed, [1] predictor ms, [2] cell + neighbour-list ms,
+ *   [0] passes timed, [1] predictor ms, [2] cell + neighbour-list ms,
  *   [3] force kernel ms (the sweep kernel alone), [4] role sum + update ms,
  *   [5] accepted triplets summed over the timed steps. */
 tb3_status tb3_set_profiling(tb3_ctx *ctx, int enable);
@@ -190,6 +195,13 @@ tb3_status tb3_get_profile(const tb3_ctx *ctx, double *out, int64_t n);
  * No counterpart in the reference. */
 tb3_status tb3_selftest_fp64(tb3_ctx *ctx, int64_t n, const double *a, const double *b, double *sqrt_a,
                              double *a_div_b, double *rsq_a);
+
+/* Diagnostic: the bit-exact path's in-range sequences for sqrt(a) and a / b
+ * (the compiler's IEEE expansions without their range-scaling steps, used on
+ * operands within 2^-500 .. 2^500), element-wise, for checking that they give
+ * the IEEE results' bits on such operands.  No counterpart in the reference. */
+tb3_status tb3_selftest_fp64_inrange(tb3_ctx *ctx, int64_t n, const double *a, const double *b, double *sqrt_a,
+                                     double *a_div_b);
 
 #ifdef __cplusplus
 }

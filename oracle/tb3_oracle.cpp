@@ -763,8 +763,11 @@ int orc_simulate(int64_t n, double *pos, double *vel, double *acc, const double 
     return 0;
 }
 
-double orc_energy(int64_t n, const double *pos, double nu, double cutoff, const double *L, uint64_t *count,
-                  int nthreads)
+/* The textbook energy sum over the accepted triplets; *scale, if non-NULL,
+ * receives the sum over the same triplets of |nu| (1 + |3 cos cos cos|) / r^3,
+ * the lengths of each term's two parts. */
+static double energy_sum(int64_t n, const double *pos, double nu, double cutoff, const double *L, uint64_t *count,
+                         double *scale, int nthreads)
 {
     nthreads = resolve_threads(nthreads);
     const double Lz[3] = {-1, -1, -1};
@@ -772,9 +775,9 @@ double orc_energy(int64_t n, const double *pos, double nu, double cutoff, const 
     std::vector<std::vector<int>> nb;
     neighbour_lists(n, pos, cutoff, L, nb, nthreads);
     const double c2 = cutoff * cutoff;
-    double E = 0.0;
+    double E = 0.0, Es = 0.0;
     uint64_t cnt = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : E, cnt) num_threads(nthreads)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : E, Es, cnt) num_threads(nthreads)
     for (int64_t p = 0; p < n; ++p) {
         const std::vector<int> &l = nb[(size_t)p];
         const size_t split = (size_t)(std::lower_bound(l.begin(), l.end(), (int)p) - l.begin());
@@ -797,12 +800,74 @@ double orc_energy(int64_t n, const double *pos, double nu, double cutoff, const 
                 const double cosk = (ik[0] * jk[0] + ik[1] * jk[1] + ik[2] * jk[2]) / (rik * rjk);
                 const double r = rij * rik * rjk;
                 E += nu * (1.0 + 3.0 * cosi * cosj * cosk) / (r * r * r);
+                Es += std::fabs(nu) * (1.0 + std::fabs(3.0 * cosi * cosj * cosk)) / (r * r * r);
                 cnt++;
             }
         }
     }
     if (count) *count = cnt;
+    if (scale) *scale = Es;
     return E;
+}
+
+double orc_energy(int64_t n, const double *pos, double nu, double cutoff, const double *L, uint64_t *count,
+                  int nthreads)
+{
+    return energy_sum(n, pos, nu, cutoff, L, count, nullptr, nthreads);
+}
+
+double orc_energy_scale(int64_t n, const double *pos, double nu, double cutoff, const double *L, uint64_t *count,
+                        int nthreads)
+{
+    double s = 0.0;
+    energy_sum(n, pos, nu, cutoff, L, count, &s, nthreads);
+    return s;
+}
+
+double orc_force_scale(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *scale,
+                       int nthreads)
+{
+    /* The same three role streams per particle as orc_sweep_owner; every
+     * contribution the reference subtracts from p's force (AxilrodTeller.cpp
+     * 83-93) adds its Euclidean length to scale[p]. */
+    nthreads = resolve_threads(nthreads);
+    const double Lz[3] = {-1, -1, -1};
+    if (!L) L = Lz;
+    std::vector<std::vector<int>> nb;
+    neighbour_lists(n, pos, cutoff, L, nb, nthreads);
+    const double c2 = cutoff * cutoff;
+    double top = 0.0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : top) num_threads(nthreads)
+    for (int64_t p = 0; p < n; ++p) {
+        const std::vector<int> &l = nb[(size_t)p];
+        const size_t split = (size_t)(std::lower_bound(l.begin(), l.end(), (int)p) - l.begin());
+        const double *P = pos + 3 * p;
+        double ci[3], cj[3], ck[3];
+        double s = 0.0;
+        auto len3 = [](const double *c) { return std::sqrt((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2]); };
+        for (size_t a = 0; a < l.size(); ++a) {
+            const int q = l[a];
+            const double *Q = pos + 3 * q;
+            for (size_t b = a + 1; b < l.size(); ++b) {
+                const int r = l[b];
+                if (!pair_ok(pos, q, r, cutoff, c2, L)) continue;
+                const double *R = pos + 3 * r;
+                if (a >= split) { /* p < q < r: p is i */
+                    at_contrib_xyz(P[0], P[1], P[2], Q[0], Q[1], Q[2], R[0], R[1], R[2], nu, ci, cj, ck);
+                    s += len3(ci);
+                } else if (b >= split) { /* q < p < r: p is j */
+                    at_contrib_xyz(Q[0], Q[1], Q[2], P[0], P[1], P[2], R[0], R[1], R[2], nu, ci, cj, ck);
+                    s += len3(cj);
+                } else { /* q < r < p: p is k */
+                    at_contrib_xyz(Q[0], Q[1], Q[2], R[0], R[1], R[2], P[0], P[1], P[2], nu, ci, cj, ck);
+                    s += len3(ck);
+                }
+            }
+        }
+        scale[p] = s;
+        top = s > top ? s : top;
+    }
+    return top;
 }
 
 double orc_forces_fast(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f,

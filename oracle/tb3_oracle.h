@@ -100,6 +100,23 @@ int orc_simulate(int64_t n, double *pos, double *vel, double *acc, const double 
 double orc_energy(int64_t n, const double *pos, double nu, double cutoff, const double *L, uint64_t *count,
                   int nthreads);
 
+/* The condition scale of that energy sum: the sum over the same triplets of
+ * |nu| (1 + |3 cos g1 cos g2 cos g3|) / (r12 r13 r23)^3, the lengths of each
+ * term's two parts (the error bound of a sum whose terms carry relative
+ * error e is e times this).  At least |E|, equal when every term's cosine
+ * product is non-negative and nu > 0. */
+double orc_energy_scale(int64_t n, const double *pos, double nu, double cutoff, const double *L, uint64_t *count,
+                        int nthreads);
+
+/* Per-particle force scale: scale[p] = sum over the accepted triplets that
+ * contain p of the Euclidean length of the contribution the reference
+ * subtracts from p's force (AxilrodTeller.cpp:83-93).  By the triangle
+ * inequality scale[p] >= |f_p|, with equality for a single triplet; it is
+ * the condition scale of p's force sum, the denominator of the mixed-precision
+ * error metric.  Returns the largest scale. */
+double orc_force_scale(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *scale,
+                       int nthreads);
+
 /* Gradient (the reference's f) and energy from the (u,v,w) squared-distance
  * formulation the GPU fast kernels use, owner-computes, FP64. */
 double orc_forces_fast(int64_t n, const double *pos, double nu, double cutoff, const double *L, double *f,

@@ -56,6 +56,11 @@ def lib():
         L.orc_forces_fast.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _u64p,
                                       ctypes.c_int]
         L.orc_forces_fast.restype = ctypes.c_double
+        L.orc_energy_scale.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _u64p,
+                                       ctypes.c_int]
+        L.orc_energy_scale.restype = ctypes.c_double
+        L.orc_force_scale.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
+        L.orc_force_scale.restype = ctypes.c_double
         L.orc_sweep_cells_fast.argtypes = [ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _u64p,
                                            ctypes.c_int]
         L.orc_sweep_cells_fast.restype = ctypes.c_double
@@ -229,6 +234,25 @@ def forces_fast(pos, nu=1.0, cutoff=-1.0, L=None, threads=0):
     Lp = _p(_c(L)) if L is not None else None
     E = lib().orc_forces_fast(pos.shape[0], _p(pos), nu, cutoff, Lp, _p(f), ctypes.byref(cnt), threads)
     return f, E, int(cnt.value)
+
+
+def energy_scale(pos, nu=1.0, cutoff=-1.0, L=None, threads=0):
+    """Sum over the triplets of |nu| (1 + |3 cos cos cos|) / r^3: the
+    condition scale of the energy sum."""
+    pos = _c(pos)
+    cnt = ctypes.c_uint64(0)
+    Lp = _p(_c(L)) if L is not None else None
+    return lib().orc_energy_scale(pos.shape[0], _p(pos), nu, cutoff, Lp, ctypes.byref(cnt), threads)
+
+
+def force_scale(pos, nu=1.0, cutoff=-1.0, L=None, threads=0):
+    """Per particle, the summed lengths of the reference's triplet
+    contributions to its force (the condition scale of its force sum)."""
+    pos = _c(pos)
+    s = np.zeros(pos.shape[0])
+    Lp = _p(_c(L)) if L is not None else None
+    lib().orc_force_scale(pos.shape[0], _p(pos), nu, cutoff, Lp, _p(s), threads)
+    return s
 
 
 def sweep_cells_fast(pos, nu=1.0, cutoff=2.0, L=None, threads=0):

@@ -116,6 +116,63 @@ def test_nu_gradient_central_difference(eng):
     assert float(nu.grad) == pytest.approx(fd, rel=1e-7)
 
 
+def test_nu_derivatives_at_zero(eng, oracle):
+    """E and dE/dx are linear in nu, so at nu = 0 (where both vanish) dE/dnu
+    is the energy at nu = 1 and d(dE/dx)/dnu is dE/dx at nu = 1: checked
+    against the oracle and against central differences in nu, through the C
+    ABI (tb3_energy_grad) and through both autograd functions."""
+    from threebodymdalgorithms_amd.autograd import at_energy, at_forces
+    P = jittered_grid(7, 0.8, 0.05, 6)
+    L = np.full(3, 5.6)
+    E1, _ = oracle.energy(P, nu=1.0, cutoff=2.5, L=L)
+    *_, f1, _ = oracle.sweep(P, nu=1.0, cutoff=2.5, L=L, mode="owner")
+    eng.set_particles(P)
+    eng.configure("p3bca", nu=0.0, cutoff=2.5, domain_L=L, precision="fp64")
+    E0, g0, dnu, _ = eng.energy_grad()
+    assert E0 == 0.0 and not np.any(g0)
+    assert dnu == pytest.approx(E1, rel=1e-12)
+    h = 1e-3
+    eng.configure("p3bca", nu=h, cutoff=2.5, domain_L=L, precision="fp64")
+    Ep, _ = eng.energy()
+    eng.configure("p3bca", nu=-h, cutoff=2.5, domain_L=L, precision="fp64")
+    Em, _ = eng.energy()
+    assert dnu == pytest.approx((Ep - Em) / (2 * h), rel=1e-10)
+
+    eng.configure("p3bca", nu=1.0, cutoff=2.5, domain_L=L, precision="fp64")
+    x = torch.tensor(P, dtype=torch.float64, device="cuda", requires_grad=True)
+    nu = torch.tensor(0.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    E = at_energy(x, nu, eng)
+    E.backward()
+    assert float(E.detach()) == 0.0 and not torch.any(x.grad)
+    assert float(nu.grad) == pytest.approx(E1, rel=1e-12)
+
+    T = torch.tensor(np.random.default_rng(1).normal(size=P.shape), dtype=torch.float64, device="cuda")
+    nu2 = torch.tensor(0.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    (at_forces(x.detach(), nu2, eng) * T).sum().backward()
+    want = float(np.sum(f1 * T.cpu().numpy()))
+    assert float(nu2.grad) == pytest.approx(want, rel=1e-10, abs=1e-10 * float(np.sum(np.abs(f1))))
+
+
+def test_energy_forward_carries_the_gradient(eng):
+    """When pos requires a gradient, at_energy's forward runs one adjoint
+    sweep and backward reuses its dE/dx: one force pass per forward/backward,
+    and the gradient equals the separate adjoint call's."""
+    from threebodymdalgorithms_amd.autograd import at_energy
+    P = jittered_grid(8, 0.8, 0.05, 8)
+    L = np.full(3, 6.4)
+    eng.set_particles(P)
+    eng.configure("p3bca", cutoff=2.5, domain_L=L, precision="exact")
+    _, g_ref, _, _ = eng.energy_grad()
+    x = torch.tensor(P, dtype=torch.float64, device="cuda", requires_grad=True)
+    eng.set_profiling(True)
+    E = at_energy(x, 1.0, eng)
+    E.backward()
+    passes = eng.get_profile()["steps"]
+    eng.set_profiling(False)
+    assert np.array_equal(x.grad.cpu().numpy(), g_ref)
+    assert passes == 1, passes
+
+
 def test_relaxation_lowers_energy(eng, oracle):
     """Gradient descent x <- x - lr dE/dx.  On the exact path dE/dx is the
     reference's f bit for bit, so the GPU trajectory equals the same descent

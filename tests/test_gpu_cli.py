@@ -58,7 +58,9 @@ def test_cli_matches_oracle_cli(tmp_path, algo, n, box, cutoff, steps, full):
 @pytest.mark.parametrize("algo,n,box,cutoff,steps", [("p3bca", 600, 9.0, 2.0, 3), ("auta", 120, 4.0, 0.0, 2)])
 def test_cli_profile_json(tmp_path, algo, n, box, cutoff, steps):
     """-op writes the PROFILE_3BMDA JSON (main.cpp:127-451) at one process:
-    per-step sweep times under "calculateForces" and "calculateInteractions",
+    per-step times under "SumUpParticles" (the pass's last kernel: the role
+    sum and update), "calculateForces" and "calculateInteractions" (the sweep),
+    in the reference's key order,
     the hitrate with the reference's averaging (step 0's accepted/candidate
     ratio over the number of recorded hitrates), one "0" member per step in
     "hitrate per proc", and a zero "stvr"."""
@@ -80,7 +82,8 @@ def test_cli_profile_json(tmp_path, algo, n, box, cutoff, steps):
     assert [k for k, _ in doc] == ["times", "hitrate", "hitrate per proc", "stvr"]
     d = dict(doc)
     times = dict(d["times"])
-    assert list(times) == ["calculateForces", "calculateInteractions"]
+    assert list(times) == ["SumUpParticles", "calculateForces", "calculateInteractions"]
+    assert dict(times["calculateForces"]) == dict(times["calculateInteractions"])
     for key in times:
         t = dict(times[key])
         assert [k for k, _ in times[key]] == ["time unit", "avg time all proc", "total time all proc",
@@ -98,3 +101,24 @@ def test_cli_profile_json(tmp_path, algo, n, box, cutoff, steps):
     assert [k for k, _ in d["hitrate per proc"]] == ["0"] * steps
     assert [v for _, v in d["hitrate per proc"]] == [pytest.approx(h, rel=1e-15) for h in hr]
     assert [dict(x) for x in d["stvr"]] == [{"rank": [0], "stvr": 0.0}]
+
+
+def test_cli_timing_lines(tmp_path):
+    """--timing prints the MEASURESIMSTEP_3BMDA line of Simulation.cpp:50-72
+    after every step, in its exact shape: {"allTimes": [ns], "max": ns} with
+    one entry per process (one here), max equal to it, and ", " separators."""
+    import json
+    import re
+    src = tmp_path / "particles.csv"
+    generators.main([str(v) for v in ("--generator", "uniform", "--numparticles", 400, "--seed0", 5, "--blx", 8,
+                                       "--bly", 8, "--blz", 8, "--mass", 1, "--output", src)])
+    say("cli --timing p3bca n=400 steps=4")
+    g = run(tb3.CLI_PATH, "-a", "p3bca", "-i", 4, "-d", 0.001, "-gy", -9.81, "-c", 2.0, "-csv", src, "--timing")
+    assert g.returncode == 0, g.stderr
+    lines = [ln for ln in g.stdout.splitlines() if ln.startswith("{")]
+    assert len(lines) == 4
+    for ln in lines:
+        assert re.fullmatch(r'\{"allTimes": \[\d+\], "max": \d+\}', ln), ln
+        d = json.loads(ln)
+        assert list(d) == ["allTimes", "max"] and len(d["allTimes"]) == 1
+        assert d["max"] == d["allTimes"][0] > 0

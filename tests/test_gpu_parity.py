@@ -77,6 +77,41 @@ def test_fp64_sqrt_and_division_round_correctly(eng):
     assert rel.max() < 1e-6
 
 
+def test_fp64_inrange_sequences_match_ieee(eng):
+    """The exact path's division and square root without the range-scaling
+    steps give the IEEE results' bits on operands within 2^-500 .. 2^500: the
+    numerators (small integers, 5 d, 3 nu) and denominators (products of up to
+    the 11th power of a side) the brackets form, random mantissas with
+    exponents across that range, values next to powers of two, and all-ones
+    mantissas."""
+    rng = np.random.default_rng(1)
+    n = 1_000_000
+    # a and b within 2^-250 .. 2^250, so that a / b is within 2^-500 .. 2^500
+    a = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-250, 250, n)) * rng.choice([-1.0, 1.0], n)
+    b = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-250, 250, n)) * rng.choice([-1.0, 1.0], n)
+    d = 2.0 ** rng.uniform(-40, 40, n)
+    small = rng.choice([-8.0, 1.0, 3.0, 5.0, 6.0], n)
+    pw = rng.integers(1, 12, n)
+    a2 = np.where(rng.random(n) < 0.5, small, 5.0 * d)
+    b2 = (d ** pw) * 2.0 ** rng.uniform(-1, 1, n)
+    e = rng.integers(-245, 245, 200_000).astype(float)
+    edge = np.concatenate([2.0 ** e, np.nextafter(2.0 ** e, 0), np.nextafter(2.0 ** e, np.inf),
+                           (2.0 - 2.0 ** -52) * 2.0 ** e])
+    A = np.concatenate([a, a2, edge, rng.permutation(edge)])
+    B = np.concatenate([b, b2, rng.permutation(edge), edge])
+    say(f"in-range sequences n={A.size}")
+    # square roots over 2^-500 .. 2^500
+    S = np.concatenate([np.abs(A), np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-500, 500, n))])
+    sq, _ = eng.selftest_fp64_inrange(S, np.ones_like(S))
+    ref_sq, _, _ = eng.selftest_fp64(S, np.ones_like(S))
+    assert np.array_equal(sq, ref_sq), int(np.sum(sq != ref_sq))
+    assert np.array_equal(sq, np.sqrt(S))
+    _, q = eng.selftest_fp64_inrange(A, B)
+    _, ref_q, _ = eng.selftest_fp64(A, B)
+    assert np.array_equal(q, ref_q), int(np.sum(q != ref_q))
+    assert np.array_equal(q, A / B)
+
+
 @pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 150, 4.0, 0.0), ("nata", 97, 4.0, 0.0),
                                                ("p3bca", 800, 9.0, 2.0), ("p3bca", 300, 5.0, 0.0),
                                                ("p3bca", 400, 4.5, 2.4)])

@@ -323,6 +323,86 @@ def test_close_pairs_fast_form_more_accurate_than_reference(oracle):
     assert worst_ref >= 10 * worst_fast
 
 
+def test_reordered_fp64_drifts_over_ten_steps(oracle):
+    """Why the bit-exact path exists beside the fast FP64 one.  Ten steps of
+    P3BCA (c = 2, density 2, N = 4000) with the reference's expression and
+    order, and the same ten steps with the squared-distance form (the fast
+    kernels' formulation, another summation order): the first sweep agrees to
+    1e-13 of the RMS force and the counts agree throughout, but the chaotic
+    dynamics (close pairs get forces ~1e7 and fly off) amplify those ulp-level
+    differences by ~10x per step, past 1e-10 by the last steps.  No reordering of
+    the reference's sums can therefore hold a 1e-12 bar over ten steps."""
+    n = 4000
+    P = uniform_particles(n, (n / 2.0) ** (1 / 3), 77)
+    L, _ = oracle.domain_size(P)
+    g, dt, m = np.array([0.0, -9.81, 0.0]), 0.001, np.ones(n)
+    xa, va, aa = P.copy(), np.zeros_like(P), np.zeros_like(P)
+    xb, vb, ab = P.copy(), np.zeros_like(P), np.zeros_like(P)
+    floored = []
+    for _ in range(10):
+        xa = oracle.predictor(xa, va, aa, dt)
+        xb = oracle.predictor(xb, vb, ab, dt)
+        *_, fa, ca = oracle.sweep(xa, cutoff=2.0, L=L, mode="owner")
+        fb, _, cb = oracle.forces_fast(xb, 1.0, 2.0, L)
+        assert ca == cb
+        va, aa = oracle.update(fa, m, va, aa, dt, g)
+        vb, ab = oracle.update(fb, m, vb, ab, dt, g)
+        num = np.linalg.norm(fb - fa, axis=1)
+        den = np.linalg.norm(fa, axis=1)
+        floored.append((num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max())
+    assert floored[0] <= 1e-12
+    assert max(floored[-3:]) >= 1e-10
+
+
+def test_force_scale(oracle):
+    """scale[p] = sum over p's triplets of |contribution to p|: equal to |f_p|
+    for a single triplet (every role), at least |f_p| in general (triangle
+    inequality), exactly linear in nu, and equal to the sum written out over
+    every triplet by hand for a small periodic system."""
+    rng = np.random.default_rng(12)
+    for trial in range(10):
+        P = rng.uniform(0, 2, (3, 3))
+        *_, f, _ = oracle.sweep(P, mode="brute")
+        s = oracle.force_scale(P)
+        assert np.allclose(s, np.linalg.norm(f, axis=1), rtol=1e-15, atol=0)
+    P = uniform_particles(60, 3.0, 4)
+    L, _ = oracle.domain_size(P)
+    *_, f, _ = oracle.sweep(P, cutoff=1.5, L=L, mode="owner")
+    s = oracle.force_scale(P, cutoff=1.5, L=L)
+    assert np.all(s >= np.linalg.norm(f, axis=1) * (1 - 1e-15))
+    assert np.array_equal(oracle.force_scale(P, nu=2.0, cutoff=1.5, L=L), 2.0 * s)
+    hand = np.zeros(len(P))
+    for i in range(len(P)):
+        for j in range(i + 1, len(P)):
+            for k in range(j + 1, len(P)):
+                if any(oracle.sqr_dist_periodic(P[a], P[b], L) > 1.5 ** 2 for a, b in ((i, j), (i, k), (j, k))):
+                    continue
+                for who, c in zip((i, j, k), oracle.at_contrib(P[i], P[j], P[k])):
+                    hand[who] += np.linalg.norm(c)
+    assert np.allclose(s, hand, rtol=1e-13, atol=0)
+    assert hand.max() > 0
+
+
+def test_energy_scale(oracle):
+    """sum |nu| (1 + |3 cos cos cos|) / r^3: equal to E for an equilateral
+    triplet (all cosines 1/2), -2E = nu / (2 a^9) for an evenly spaced
+    collinear one (cosines 1, 1, -1), at least |E| in general, linear in nu."""
+    a = 1.1
+    P = np.array([[0, 0, 0], [a, 0, 0], [a / 2, a * math.sqrt(3) / 2, 0]], float) @ rot(3).T
+    E, _ = oracle.energy(P)
+    assert oracle.energy_scale(P) == pytest.approx(E, rel=1e-14)
+    P = np.array([[0, 0, 0], [a, 0, 0], [2 * a, 0, 0]], float) @ rot(4).T
+    E, _ = oracle.energy(P)
+    s = oracle.energy_scale(P)
+    assert s == pytest.approx(-2 * E, rel=1e-14) and s == pytest.approx(1 / (2 * a ** 9), rel=1e-14)
+    P = uniform_particles(200, 4.0, 8)
+    L, _ = oracle.domain_size(P)
+    E, _ = oracle.energy(P, cutoff=1.8, L=L)
+    s = oracle.energy_scale(P, cutoff=1.8, L=L)
+    assert s >= abs(E) and s > 0
+    assert oracle.energy_scale(P, nu=3.0, cutoff=1.8, L=L) == pytest.approx(3 * s, rel=1e-14)
+
+
 def _unique_rank_triplets(p):
     """generateAllUniqueTriplets (tests.cpp:337-348): i <= j <= k < p."""
     return sorted((i, j, k) for i in range(p) for j in range(i, p) for k in range(j, p))

@@ -65,6 +65,7 @@ SIGNATURES = {
     "tb3_set_profiling": (ctypes.c_int, [_ctxp, ctypes.c_int]),
     "tb3_get_profile": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64]),
     "tb3_selftest_fp64": (ctypes.c_int, [_ctxp, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]),
+    "tb3_selftest_fp64_inrange": (ctypes.c_int, [_ctxp, ctypes.c_int64, _dp, _dp, _dp, _dp]),
 }
 
 _lib = None

@@ -1,16 +1,18 @@
 """torch.autograd bindings of the energy forward and its hand-written adjoint.
 
     E = at_energy(pos, nu, engine)      pos: (N, 3) float64 CUDA tensor, nu: 0-d float64 tensor
-    E.backward()                        pos.grad = dE/dx (the reference's f), nu.grad = dE/dnu = E / nu
+    E.backward()                        pos.grad = dE/dx (the reference's f), nu.grad = dE/dnu
 
-The forward launches the energy-only sweep; the backward launches the adjoint
-sweep (energy + dE/dx) on the same positions.  Both run on the engine's GPU
-through the C ABI on torch's current stream; the engine's algorithm, cutoff,
-domain and precision apply.
+When pos requires a gradient the forward runs one adjoint sweep, which gives
+E and dE/dx together, and the backward only scales the saved dE/dx; otherwise
+the forward runs the energy-only sweep.  E is linear in nu, so dE/dnu is the
+energy at nu = 1: E / nu, or at nu = 0 an energy sweep at nu = 1.  The sweeps
+run on the engine's GPU through the C ABI on torch's current stream; the
+engine's algorithm, cutoff, domain and precision apply.
 
 `at_forces(pos, nu, engine)` returns dE/dx itself with a gradient for nu only
-(dE/dx is linear in nu, so d(dE/dx)/dnu = (dE/dx) / nu); it serves force
-matching fits of nu.
+(dE/dx is linear in nu, so d(dE/dx)/dnu = (dE/dx) / nu, or dE/dx at nu = 1
+when nu = 0); it serves force matching fits of nu.
 """
 import torch
 
@@ -35,23 +37,41 @@ class _Energy(torch.autograd.Function):
         pos_c = pos.detach().contiguous()
         nu_v = float(nu)
         E = torch.zeros((), dtype=torch.float64, device=pos.device)
-        engine.energy_dev(pos_c.shape[0], pos_c.data_ptr(), nu_v, E.data_ptr(), _stream_handle(pos.device))
+        grad = None
+        if ctx.needs_input_grad[0]:
+            # one sweep gives E and dE/dx together; backward only scales it
+            grad = torch.empty_like(pos_c)
+            engine.energy_grad_dev(pos_c.shape[0], pos_c.data_ptr(), nu_v, E.data_ptr(), grad.data_ptr(),
+                                   _stream_handle(pos.device))
+        else:
+            engine.energy_dev(pos_c.shape[0], pos_c.data_ptr(), nu_v, E.data_ptr(), _stream_handle(pos.device))
         ctx.engine = engine
         ctx.nu_v = nu_v
-        ctx.save_for_backward(pos_c)
+        ctx.has_grad = grad is not None
+        ctx.save_for_backward(pos_c, E.detach().clone(), grad if grad is not None else E.new_zeros(()))
         return E
 
     @staticmethod
     def backward(ctx, gE):
-        (pos_c,) = ctx.saved_tensors
-        E = torch.zeros((), dtype=torch.float64, device=pos_c.device)
-        grad = torch.empty_like(pos_c)
-        ctx.engine.energy_grad_dev(pos_c.shape[0], pos_c.data_ptr(), ctx.nu_v, E.data_ptr(), grad.data_ptr(),
-                                   _stream_handle(pos_c.device))
-        g_pos = grad * gE if ctx.needs_input_grad[0] else None
+        pos_c, E, grad = ctx.saved_tensors
+        g_pos = None
+        if ctx.needs_input_grad[0]:
+            if not ctx.has_grad:
+                grad = torch.empty_like(pos_c)
+                E2 = torch.zeros((), dtype=torch.float64, device=pos_c.device)
+                ctx.engine.energy_grad_dev(pos_c.shape[0], pos_c.data_ptr(), ctx.nu_v, E2.data_ptr(),
+                                           grad.data_ptr(), _stream_handle(pos_c.device))
+            g_pos = grad * gE
         g_nu = None
         if ctx.needs_input_grad[1]:
-            g_nu = (E / ctx.nu_v) * gE if ctx.nu_v != 0.0 else torch.zeros_like(E)
+            # E is linear in nu: dE/dnu is the energy at nu = 1
+            if ctx.nu_v != 0.0:
+                g_nu = (E / ctx.nu_v) * gE
+            else:
+                E1 = torch.zeros((), dtype=torch.float64, device=pos_c.device)
+                ctx.engine.energy_dev(pos_c.shape[0], pos_c.data_ptr(), 1.0, E1.data_ptr(),
+                                      _stream_handle(pos_c.device))
+                g_nu = E1 * gE
         return g_pos, g_nu, None
 
 
@@ -66,18 +86,26 @@ class _Forces(torch.autograd.Function):
         engine.energy_grad_dev(pos_c.shape[0], pos_c.data_ptr(), nu_v, E.data_ptr(), grad.data_ptr(),
                                _stream_handle(pos.device))
         ctx.nu_v = nu_v
-        ctx.save_for_backward(grad)
+        ctx.engine = engine
+        ctx.save_for_backward(grad, pos_c)
         return grad
 
     @staticmethod
     def backward(ctx, gF):
-        (grad,) = ctx.saved_tensors
+        grad, pos_c = ctx.saved_tensors
         if ctx.needs_input_grad[0]:
             raise RuntimeError("at_forces has no second derivative with respect to positions")
         g_nu = None
         if ctx.needs_input_grad[1]:
-            g_nu = (gF * grad).sum() / ctx.nu_v if ctx.nu_v != 0.0 else torch.zeros((), dtype=grad.dtype,
-                                                                                   device=grad.device)
+            # dE/dx is linear in nu: its nu-derivative is dE/dx at nu = 1
+            if ctx.nu_v != 0.0:
+                g_nu = (gF * grad).sum() / ctx.nu_v
+            else:
+                E1 = torch.zeros((), dtype=torch.float64, device=pos_c.device)
+                grad1 = torch.empty_like(pos_c)
+                ctx.engine.energy_grad_dev(pos_c.shape[0], pos_c.data_ptr(), 1.0, E1.data_ptr(), grad1.data_ptr(),
+                                           _stream_handle(pos_c.device))
+                g_nu = (gF * grad1).sum()
         return None, g_nu, None
 
 

@@ -54,10 +54,52 @@ __device__ __forceinline__ double sqr_dist_periodic_min(double ax, double ay, do
     return ex * ex + ey * ey + ez * ez;
 }
 
-// Eigen's Vector3d::norm() as sqrt((x*x + y*y) + z*z) (AxilrodTeller.cpp:29-31).
-__device__ __forceinline__ double ref_norm3(double x, double y, double z)
+// Correctly rounded FP64 division and square root for operands in range.
+//
+// gfx950 has no FP64 divide or square-root instruction; the compiler expands
+// a / b into a Newton-Raphson sequence (reciprocal estimate, two refinements
+// y <- y + y (1 - b y), quotient q = a y, and the Markstein correction
+// q + y (a - b q), each step one FMA) wrapped in three range steps:
+// v_div_scale scales an operand by a power of two when the quotient or the
+// reciprocal would leave the normal range or the numerator is near the
+// bottom of it, v_div_fmas undoes that scaling in the last FMA, and
+// v_div_fixup handles zeros, infinities, NaNs and overflow.  For finite
+// operands far inside the normal range (every caller here keeps numerators,
+// denominators and quotients within 2^-500 .. 2^500, or a zero numerator)
+// the scale step leaves both operands as they are, the last FMA is a plain
+// FMA, and the fixup returns the quotient unchanged.  div_rn_inrange is the
+// sequence without the three range steps: on such operands it yields the
+// bits of a / b in 8 instructions instead of 11.  The square root likewise:
+// sqrt(x) rescales x when it is tiny and passes +-0 and +inf through around
+// a Newton iteration on the reciprocal square-root estimate; for x within
+// 2^-500 .. 2^500 those steps are the identity and sqrt_rn_inrange is the
+// iteration alone (10 instructions instead of 17).  Both are checked bit for
+// bit against the compiler's IEEE expansions on the device
+// (tb3_selftest_fp64_fast).
+__device__ __forceinline__ double div_rn_inrange(double a, double b)
 {
-    return __builtin_sqrt((x * x + y * y) + z * z);
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double q = a * r;
+    const double rem = __builtin_fma(-b, q, a);
+    return __builtin_fma(rem, r, q);
+}
+
+__device__ __forceinline__ double sqrt_rn_inrange(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
 }
 
 struct RefPowers {
@@ -76,24 +118,106 @@ __device__ __forceinline__ RefPowers ref_powers(double d1)
     return p;
 }
 
+// IEEE division, or its in-range sequence when the caller has checked the
+// operands' range.
+template <bool INRANGE>
+__device__ __forceinline__ double ref_div(double a, double b)
+{
+    if constexpr (INRANGE) return div_rn_inrange(a, b);
+    else return a / b;
+}
+
 // One dVdR bracket (AxilrodTeller.cpp:68-81) for own side x and the two other
 // sides y, z in the order the source names them: dVdRa = (a; b, c),
 // dVdRb = (b; a, c), dVdRc = (c; b, a).  Left-to-right sums and products.
+template <bool INRANGE = false>
 __device__ __forceinline__ double ref_bracket(double nu, const RefPowers &x, const RefPowers &y,
                                               const RefPowers &z)
 {
-    double t = -8. / (x.d4 * y.d3 * z.d3);
-    t = t - 1. / (y.d5 * z.d5);
-    t = t + 5. * y.d1 / (x.d6 * z.d5);
-    t = t + 5. * z.d1 / (x.d6 * y.d5);
-    t = t - 1. / (x.d2 * y.d3 * z.d5);
-    t = t - 1. / (x.d2 * y.d5 * z.d3);
-    t = t - 3. / (x.d4 * y.d1 * z.d5);
-    t = t - 3. / (x.d4 * y.d5 * z.d1);
-    t = t - 5. / (x.d6 * y.d1 * z.d3);
-    t = t - 5. / (x.d6 * y.d3 * z.d1);
-    t = t + 6. / (x.d4 * y.d3 * z.d3);
-    return (3. * nu / (8. * x.d1)) * t;
+    double t = ref_div<INRANGE>(-8., x.d4 * y.d3 * z.d3);
+    t = t - ref_div<INRANGE>(1., y.d5 * z.d5);
+    t = t + ref_div<INRANGE>(5. * y.d1, x.d6 * z.d5);
+    t = t + ref_div<INRANGE>(5. * z.d1, x.d6 * y.d5);
+    t = t - ref_div<INRANGE>(1., x.d2 * y.d3 * z.d5);
+    t = t - ref_div<INRANGE>(1., x.d2 * y.d5 * z.d3);
+    t = t - ref_div<INRANGE>(3., x.d4 * y.d1 * z.d5);
+    t = t - ref_div<INRANGE>(3., x.d4 * y.d5 * z.d1);
+    t = t - ref_div<INRANGE>(5., x.d6 * y.d1 * z.d3);
+    t = t - ref_div<INRANGE>(5., x.d6 * y.d3 * z.d1);
+    t = t + ref_div<INRANGE>(6., x.d4 * y.d3 * z.d3);
+    return ref_div<INRANGE>(3. * nu, 8. * x.d1) * t;
+}
+
+// The range the in-range sequences need, per triplet: every squared side in
+// [2^-80, 2^80] puts the sides in [2^-40, 2^40], so every product of powers
+// in the brackets (at most the 11th power of a side in total) lies within
+// 2^-440 .. 2^440, every quotient within 2^-500 .. 2^500 and the square
+// roots' arguments within 2^-80 .. 2^80; nu must be 0 or within 2^-200 ..
+// 2^200 (nu_inrange, checked once per launch), which keeps 3 nu / (8 d1)
+// within 2^-250 .. 2^250.
+__device__ __forceinline__ bool sides_inrange(double sa, double sb, double sc)
+{
+    constexpr double lo = 0x1p-80, hi = 0x1p80;
+    return sa >= lo && sa <= hi && sb >= lo && sb <= hi && sc >= lo && sc <= hi;
+}
+
+__host__ __device__ inline bool nu_inrange(double nu)
+{
+    const double m = nu < 0 ? -nu : nu;
+    return nu == 0.0 || (m >= 0x1p-200 && m <= 0x1p200);
+}
+
+// The sides of the triplet (AxilrodTeller.cpp:23-31, 53-63): the difference
+// vectors dXa = rj - ri, dXb = rk - ri, dXc = rk - rj and the squared norms
+// of a = rj - ri, b = ri - rk, c = rj - rk.  Negating a difference is exact,
+// so |b|^2 and |c|^2 from -dXb and -dXc are the reference's bits.
+struct RefSides {
+    double dXa, dYa, dZa, dXb, dYb, dZb, dXc, dYc, dZc;
+    double sa, sb, sc;
+};
+
+__device__ __forceinline__ RefSides ref_sides(double xi, double yi, double zi, double xj, double yj, double zj,
+                                              double xk, double yk, double zk)
+{
+    RefSides t;
+    t.dXa = xj - xi, t.dYa = yj - yi, t.dZa = zj - zi;
+    t.dXb = xk - xi, t.dYb = yk - yi, t.dZb = zk - zi;
+    t.dXc = xk - xj, t.dYc = yk - yj, t.dZc = zk - zj;
+    t.sa = (t.dXa * t.dXa + t.dYa * t.dYa) + t.dZa * t.dZa;
+    t.sb = (t.dXb * t.dXb + t.dYb * t.dYb) + t.dZb * t.dZb;
+    t.sc = (t.dXc * t.dXc + t.dYc * t.dYc) + t.dZc * t.dZc;
+    return t;
+}
+
+template <bool INRANGE>
+__device__ __forceinline__ double ref_sqrt(double x)
+{
+    if constexpr (INRANGE) return sqrt_rn_inrange(x);
+    else return __builtin_sqrt(x);
+}
+
+// The brackets dVdRa, dVdRb, dVdRc that role ROLE needs (-1: all three).
+template <int ROLE, bool INRANGE>
+__device__ __forceinline__ void ref_brackets(const RefSides &t, double nu, double &A, double &B, double &C)
+{
+    const RefPowers a = ref_powers(ref_sqrt<INRANGE>(t.sa));
+    const RefPowers b = ref_powers(ref_sqrt<INRANGE>(t.sb));
+    const RefPowers c = ref_powers(ref_sqrt<INRANGE>(t.sc));
+    A = B = C = 0.0;
+    if constexpr (ROLE != 2) A = ref_bracket<INRANGE>(nu, a, b, c);
+    if constexpr (ROLE != 1) B = ref_bracket<INRANGE>(nu, b, a, c);
+    if constexpr (ROLE != 0) C = ref_bracket<INRANGE>(nu, c, b, a);
+}
+
+// The brackets from the in-range sequences where the triplet's sides and nu
+// allow them (nu_ok = nu_inrange(nu), hoisted by the caller), else from the
+// IEEE expansions: the same bits either way.
+template <int ROLE>
+__device__ __forceinline__ void ref_brackets_any(const RefSides &t, double nu, bool nu_ok, double &A, double &B,
+                                                 double &C)
+{
+    if (nu_ok && sides_inrange(t.sa, t.sb, t.sc)) ref_brackets<ROLE, true>(t, nu, A, B, C);
+    else ref_brackets<ROLE, false>(t, nu, A, B, C);
 }
 
 // Contribution that AxilrodTeller::CalculateForces subtracts from the force
@@ -103,34 +227,46 @@ __device__ __forceinline__ double ref_bracket(double nu, const RefPowers &x, con
 // the result is bit-identical to the reference's.
 template <int ROLE>
 __device__ __forceinline__ void ref_contrib(double xi, double yi, double zi, double xj, double yj, double zj,
-                                            double xk, double yk, double zk, double nu, double &cx, double &cy,
-                                            double &cz)
+                                            double xk, double yk, double zk, double nu, bool nu_ok, double &cx,
+                                            double &cy, double &cz)
 {
-    const RefPowers a = ref_powers(ref_norm3(xj - xi, yj - yi, zj - zi));  // a = rj - ri
-    const RefPowers b = ref_powers(ref_norm3(xi - xk, yi - yk, zi - zk));  // b = ri - rk
-    const RefPowers c = ref_powers(ref_norm3(xj - xk, yj - yk, zj - zk));  // c = rj - rk
-    const double dXa = xj - xi, dYa = yj - yi, dZa = zj - zi;
-    const double dXb = xk - xi, dYb = yk - yi, dZb = zk - zi;
-    const double dXc = xk - xj, dYc = yk - yj, dZc = zk - zj;
+    const RefSides t = ref_sides(xi, yi, zi, xj, yj, zj, xk, yk, zk);
+    double A, B, C;
+    ref_brackets_any<ROLE>(t, nu, nu_ok, A, B, C);
     if constexpr (ROLE == 0) {
-        const double A = ref_bracket(nu, a, b, c);
-        const double B = ref_bracket(nu, b, a, c);
-        cx = dXa * A + dXb * B;
-        cy = dYa * A + dYb * B;
-        cz = dZa * A + dZb * B;
+        cx = t.dXa * A + t.dXb * B;
+        cy = t.dYa * A + t.dYb * B;
+        cz = t.dZa * A + t.dZb * B;
     } else if constexpr (ROLE == 1) {
-        const double A = ref_bracket(nu, a, b, c);
-        const double C = ref_bracket(nu, c, b, a);
-        cx = dXa * (-A) + dXc * C;
-        cy = dYa * (-A) + dYc * C;
-        cz = dZa * (-A) + dZc * C;
+        cx = t.dXa * (-A) + t.dXc * C;
+        cy = t.dYa * (-A) + t.dYc * C;
+        cz = t.dZa * (-A) + t.dZc * C;
     } else {
-        const double B = ref_bracket(nu, b, a, c);
-        const double C = ref_bracket(nu, c, b, a);
-        cx = dXb * (-B) + dXc * (-C);
-        cy = dYb * (-B) + dYc * (-C);
-        cz = dZb * (-B) + dZc * (-C);
+        cx = t.dXb * (-B) + t.dXc * (-C);
+        cy = t.dYb * (-B) + t.dYc * (-C);
+        cz = t.dZb * (-B) + t.dZc * (-C);
     }
+}
+
+// All three contributions of one triplet from one evaluation of the three
+// brackets (AxilrodTeller.cpp:65-93): ci, cj, ck as the reference subtracts
+// them from i.f, j.f and k.f.
+__device__ __forceinline__ void ref_contrib_all(double xi, double yi, double zi, double xj, double yj, double zj,
+                                                double xk, double yk, double zk, double nu, bool nu_ok, double *ci,
+                                                double *cj, double *ck)
+{
+    const RefSides t = ref_sides(xi, yi, zi, xj, yj, zj, xk, yk, zk);
+    double A, B, C;
+    ref_brackets_any<-1>(t, nu, nu_ok, A, B, C);
+    ci[0] = t.dXa * A + t.dXb * B;
+    ci[1] = t.dYa * A + t.dYb * B;
+    ci[2] = t.dZa * A + t.dZb * B;
+    cj[0] = t.dXa * (-A) + t.dXc * C;
+    cj[1] = t.dYa * (-A) + t.dYc * C;
+    cj[2] = t.dZa * (-A) + t.dZc * C;
+    ck[0] = t.dXb * (-B) + t.dXc * (-C);
+    ck[1] = t.dYb * (-B) + t.dYc * (-C);
+    ck[2] = t.dZb * (-B) + t.dZc * (-C);
 }
 
 // ---------------------------------------------------------------------------

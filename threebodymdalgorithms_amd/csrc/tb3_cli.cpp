@@ -29,9 +29,14 @@
 //                                 reference's VLEVEL line (main.cpp:561-569)
 //
 // -op <file> writes the profile JSON of a PROFILE_3BMDA build of the reference
-// (main.cpp:127-451) at one process, for the keys this path has: per-step
-// "calculateInteractions" and "calculateForces" times in ns (the sweep kernel's
-// device time: on the GPU the potential is evaluated inside the sweep), the
+// (main.cpp:127-451) at one process, for the keys this path has, in the
+// reference's key order (its timer map is ordered by name): per-step
+// "SumUpParticles" times in ns (Algorithm.cpp:237-270: the device time of the
+// kernel that ends the pass, which forms b0 + (b1 + b2) on the exact path and
+// integrates; the fast paths accumulate one force and do no separate sum, so
+// there it is the integration alone), "calculateForces" and
+// "calculateInteractions" times (the sweep kernel's device time: on the GPU
+// the potential is evaluated inside the sweep), the
 // hitrate (accepted / candidate triplets, Algorithm.cpp:92-138 counters, with
 // the reference's averaging: step 0's value over the number of recorded
 // hitrates, main.cpp:403-416, and one "0" member per step in
@@ -282,25 +287,27 @@ std::string jsonDouble(double v)
 
 // The profile JSON (see the header comment) for one process.
 bool writeProfile(const std::string &path, const std::vector<long long> &sweepNs,
-                  const std::vector<double> &hitrates)
+                  const std::vector<long long> &sumNs, const std::vector<double> &hitrates)
 {
     std::ostringstream o;
-    auto times = [&](const char *key) {
+    auto times = [&](const char *key, const std::vector<long long> &ns) {
         long long total = 0;
-        for (long long t : sweepNs) total += t;
-        const double steps = (double)sweepNs.size();
+        for (long long t : ns) total += t;
+        const double steps = (double)ns.size();
         o << "\"" << key << "\":{\"time unit\":\"nanosecond\",\"avg time all proc\":"
           << jsonDouble(steps > 0 ? (double)total / steps : 0.0) << ",\"total time all proc\":"
           << jsonDouble((double)total) << ",\"total avg time union proc\":" << jsonDouble((double)total)
           << ",\"total max time union proc\":" << total << ",\"times per proc\":{\"0\":[";
-        for (size_t i = 0; i < sweepNs.size(); ++i) o << (i ? "," : "") << sweepNs[i];
+        for (size_t i = 0; i < ns.size(); ++i) o << (i ? "," : "") << ns[i];
         o << "]}}";
     };
     o << "{\"times\":{";
     if (!sweepNs.empty()) {
-        times("calculateForces");
+        times("SumUpParticles", sumNs);
         o << ",";
-        times("calculateInteractions");
+        times("calculateForces", sweepNs);
+        o << ",";
+        times("calculateInteractions", sweepNs);
     }
     o << "},\"hitrate\":{";
     if (!hitrates.empty()) o << "\"avg hitrate per sim step\":[" << jsonDouble(hitrates[0] / (double)hitrates.size()) << "]";
@@ -347,11 +354,11 @@ int main(int argc, char **argv)
     const std::string stem = a.outputCSV.substr(0, a.outputCSV.find_last_of('.'));
     std::vector<uint64_t> counts;
     const bool profile = !a.outputProfile.empty();
-    std::vector<long long> sweepNs;
+    std::vector<long long> sweepNs, sumNs;
     std::vector<double> hitrates;
     // candidate triplets of the one-process i<j<k sweep (Algorithm.cpp:72-92)
     const double candidates = n >= 3 ? (double)n * (double)(n - 1) * (double)(n - 2) / 6.0 : 0.0;
-    double sweepMs = 0.0;
+    double sweepMs = 0.0, sumMs = 0.0;
     if (profile) {
         st = tb3_set_profiling(ctx, 1);
         if (st != TB3_OK) return die(ctx, "tb3_set_profiling", st);
@@ -368,6 +375,8 @@ int main(int argc, char **argv)
             if (st != TB3_OK) return die(ctx, "tb3_get_profile", st);
             sweepNs.push_back(std::llround((prof[3] - sweepMs) * 1e6));
             sweepMs = prof[3];
+            sumNs.push_back(std::llround((prof[4] - sumMs) * 1e6));
+            sumMs = prof[4];
             // the reference records a hitrate only when there were candidates
             if (candidates > 0) hitrates.push_back((double)nt / candidates);
         }
@@ -388,7 +397,7 @@ int main(int argc, char **argv)
             }
         }
     }
-    if (profile && !writeProfile(a.outputProfile, sweepNs, hitrates)) {
+    if (profile && !writeProfile(a.outputProfile, sweepNs, sumNs, hitrates)) {
         std::cerr << "tb3: cannot write '" << a.outputProfile << "'" << std::endl;
         tb3_destroy(ctx);
         return 1;

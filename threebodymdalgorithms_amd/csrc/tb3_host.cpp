@@ -1026,24 +1026,32 @@ tb3_status tb3_compute_forces(tb3_ctx *c, uint64_t *n_triplets)
     });
 }
 
+// The energy of the held positions at the configured nu, from a pass over a
+// copy of the positions so that the held force stays as it is.
+static double energy_on_copy(tb3_ctx *c, unsigned long long *count)
+{
+    c->scratch.ensure(c->n);
+    DeviceState sd = c->scratch.d;
+    const DeviceState &d = c->state.d;
+    if (c->n > 0) {
+        const size_t b = (size_t)c->n * sizeof(double);
+        HIPCHK(hipMemcpyAsync(sd.x, d.x, b, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(sd.y, d.y, b, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(sd.z, d.z, b, hipMemcpyDeviceToDevice, c->stream));
+    }
+    const unsigned long long cnt = run_pass(c, c->stream, sd, false, 0.0, nullptr, false, true, false);
+    if (count) *count = cnt;
+    return c->h_status->energy;
+}
+
 tb3_status tb3_energy(tb3_ctx *c, double *E, uint64_t *n_triplets)
 {
     if (!c || !E) return fail(c, TB3_EINVAL, "E is required");
     tb3_status st = check_ready(c);
     if (st != TB3_OK) return st;
     return guarded(c, [&] {
-        // the energy pass must not disturb the held force: run on a copy
-        c->scratch.ensure(c->n);
-        DeviceState sd = c->scratch.d;
-        const DeviceState &d = c->state.d;
-        if (c->n > 0) {
-            const size_t b = (size_t)c->n * sizeof(double);
-            HIPCHK(hipMemcpyAsync(sd.x, d.x, b, hipMemcpyDeviceToDevice, c->stream));
-            HIPCHK(hipMemcpyAsync(sd.y, d.y, b, hipMemcpyDeviceToDevice, c->stream));
-            HIPCHK(hipMemcpyAsync(sd.z, d.z, b, hipMemcpyDeviceToDevice, c->stream));
-        }
-        const unsigned long long cnt = run_pass(c, c->stream, sd, false, 0.0, nullptr, false, true, false);
-        *E = c->h_status->energy;
+        unsigned long long cnt = 0;
+        *E = energy_on_copy(c, &cnt);
         if (n_triplets) *n_triplets = cnt;
     });
 }
@@ -1058,7 +1066,6 @@ tb3_status tb3_energy_grad(tb3_ctx *c, double *E, double *dEdx, double *dEdnu, u
         const unsigned long long cnt = run_pass(c, c->stream, d, false, 0.0, nullptr, true, true, false);
         const double e = c->h_status->energy;
         if (E) *E = e;
-        if (dEdnu) *dEdnu = c->nu != 0.0 ? e / c->nu : 0.0;
         if (n_triplets) *n_triplets = cnt;
         if (dEdx && c->n > 0) {
             c->scratch.ensure(c->n);
@@ -1066,6 +1073,24 @@ tb3_status tb3_energy_grad(tb3_ctx *c, double *E, double *dEdx, double *dEdnu, u
             HIPCHK(hipMemcpyAsync(dEdx, c->scratch.d.facc, (size_t)c->n * 3 * sizeof(double), hipMemcpyDeviceToHost,
                                   c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        if (dEdnu) {
+            // E is linear in nu, so dE/dnu is the energy at nu = 1: E / nu,
+            // or at nu = 0 (where E / nu is undefined) a pass at nu = 1
+            if (c->nu != 0.0) {
+                *dEdnu = e / c->nu;
+            } else {
+                c->nu = 1.0;
+                double e1 = 0.0;
+                try {
+                    e1 = energy_on_copy(c, nullptr);
+                } catch (...) {
+                    c->nu = 0.0;
+                    throw;
+                }
+                c->nu = 0.0;
+                *dEdnu = e1;
+            }
         }
     });
 }
@@ -1096,7 +1121,7 @@ static tb3_status dev_common(tb3_ctx *c, int64_t n, const double *d_pos, double 
         const double saved_nu = c->nu;
         c->nu = nu;
         try {
-            run_pass(c, s, sd, false, 0.0, nullptr, grad, true, false);
+            run_pass(c, s, sd, false, 0.0, nullptr, grad, true, true);
         } catch (...) {
             c->nu = saved_nu;
             throw;
@@ -1105,6 +1130,10 @@ static tb3_status dev_common(tb3_ctx *c, int64_t n, const double *d_pos, double 
         HIPCHK(hipMemcpyAsync(d_E, &c->d_status->energy, sizeof(double), hipMemcpyDeviceToDevice, s));
         if (grad && n > 0) launch_rows_from_soa(s, (int)n, sd.fx, sd.fy, sd.fz, d_dEdx);
         HIPCHK(hipGetLastError());
+        // both reads above use the context's status block and scratch, which
+        // the context's next call (on its own stream) clears and reuses:
+        // they complete before the call returns
+        HIPCHK(hipStreamSynchronize(s));
         if (n_triplets) *n_triplets = c->h_status->count[0];
     });
 }
@@ -1233,6 +1262,32 @@ tb3_status tb3_selftest_fp64(tb3_ctx *c, int64_t n, const double *a, const doubl
             HIPCHK(hipMemcpyAsync(sqrt_a, buf + 2 * n, bytes, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipMemcpyAsync(a_div_b, buf + 3 * n, bytes, hipMemcpyDeviceToHost, c->stream));
             if (rsq_a) HIPCHK(hipMemcpyAsync(rsq_a, buf + 4 * n, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            (void)hipFree(buf);
+            throw;
+        }
+        HIPCHK(hipFree(buf));
+    });
+}
+
+tb3_status tb3_selftest_fp64_inrange(tb3_ctx *c, int64_t n, const double *a, const double *b, double *sqrt_a,
+                                     double *a_div_b)
+{
+    if (!c) return TB3_EINVAL;
+    if (n < 0 || (n > 0 && (!a || !b || !sqrt_a || !a_div_b))) return fail(c, TB3_EINVAL, "bad arguments");
+    return guarded(c, [&] {
+        if (n == 0) return;
+        double *buf = nullptr;
+        HIPCHK(hipMalloc((void **)&buf, (size_t)n * 4 * sizeof(double)));
+        try {
+            const size_t bytes = (size_t)n * sizeof(double);
+            HIPCHK(hipMemcpyAsync(buf, a, bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(buf + n, b, bytes, hipMemcpyHostToDevice, c->stream));
+            launch_selftest_fp64_inrange(c->stream, n, buf, buf + n, buf + 2 * n, buf + 3 * n);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(sqrt_a, buf + 2 * n, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(a_div_b, buf + 3 * n, bytes, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
         } catch (...) {
             (void)hipFree(buf);

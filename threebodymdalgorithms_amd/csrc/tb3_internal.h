@@ -167,6 +167,8 @@ void launch_forces_tiles(hipStream_t s, const DeviceState &d, const unsigned lon
                          int want_energy);
 
 // Element-wise sqrt(a) and a / b (diagnostic for the FP64 rounding).
+void launch_selftest_fp64_inrange(hipStream_t s, long long n, const double *a, const double *b, double *sq,
+                                  double *q);
 void launch_selftest_fp64(hipStream_t s, long long n, const double *a, const double *b, double *sq, double *q,
                           double *rs);
 

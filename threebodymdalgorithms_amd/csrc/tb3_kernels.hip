@@ -773,6 +773,8 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
     int *sh_start = reinterpret_cast<int *>(ring + RING);
     int *sh_pre = sh_start + 32;
 
+    // a halted step-graph replay leaves the step to the host
+    if (a.st->halt) return;
     const int slot = owner_slot(blockIdx.x, a.n);
     if (slot >= a.n) return;
     const int p = a.cell_idx[slot];
@@ -1657,6 +1659,7 @@ __device__ void exact_role(const double *__restrict__ x, const double *__restric
                            unsigned long long &accepted)
 {
     const int lane = lane_id();
+    const bool nu_ok = nu_inrange(nu);
     const double xp = x[p], yp = y[p], zp = z[p];
     int a0, a1, b1;
     bool tri;
@@ -1679,11 +1682,11 @@ __device__ void exact_role(const double *__restrict__ x, const double *__restric
             const double xb = x[eb], yb = y[eb], zb = z[eb];
             double cx, cy, cz;
             if constexpr (ROLE == 0)
-                ref_contrib<0>(xp, yp, zp, xa, ya, za, xb, yb, zb, nu, cx, cy, cz);
+                ref_contrib<0>(xp, yp, zp, xa, ya, za, xb, yb, zb, nu, nu_ok, cx, cy, cz);
             else if constexpr (ROLE == 1)
-                ref_contrib<1>(xa, ya, za, xp, yp, zp, xb, yb, zb, nu, cx, cy, cz);
+                ref_contrib<1>(xa, ya, za, xp, yp, zp, xb, yb, zb, nu, nu_ok, cx, cy, cz);
             else
-                ref_contrib<2>(xa, ya, za, xb, yb, zb, xp, yp, zp, nu, cx, cy, cz);
+                ref_contrib<2>(xa, ya, za, xb, yb, zb, xp, yp, zp, nu, nu_ok, cx, cy, cz);
             sh.cb[lane * 3 + 0] = cx;
             sh.cb[lane * 3 + 1] = cy;
             sh.cb[lane * 3 + 2] = cz;
@@ -1773,6 +1776,7 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
                                   unsigned long long &accepted)
 {
     const int lane = lane_id();
+    const bool nu_ok = nu_inrange(nu);
     // stream geometry: rows [s, s + m), columns after the row (triangular) or [split, len)
     int s0, m, m2;
     if constexpr (ROLE == 0) {
@@ -1801,11 +1805,11 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
             const double xb = L.px[ib], yb = L.py[ib], zb = L.pz[ib];
             double cx, cy, cz;
             if constexpr (ROLE == 0)
-                ref_contrib<0>(xp, yp, zp, xa, ya, za, xb, yb, zb, nu, cx, cy, cz);
+                ref_contrib<0>(xp, yp, zp, xa, ya, za, xb, yb, zb, nu, nu_ok, cx, cy, cz);
             else if constexpr (ROLE == 1)
-                ref_contrib<1>(xa, ya, za, xp, yp, zp, xb, yb, zb, nu, cx, cy, cz);
+                ref_contrib<1>(xa, ya, za, xp, yp, zp, xb, yb, zb, nu, nu_ok, cx, cy, cz);
             else
-                ref_contrib<2>(xa, ya, za, xb, yb, zb, xp, yp, zp, nu, cx, cy, cz);
+                ref_contrib<2>(xa, ya, za, xb, yb, zb, xp, yp, zp, nu, nu_ok, cx, cy, cz);
             sh.cb[lane * 3 + 0] = cx;
             sh.cb[lane * 3 + 1] = cy;
             sh.cb[lane * 3 + 2] = cz;
@@ -1876,6 +1880,8 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
     extern __shared__ int sid_ex[];
     __shared__ ExactShared sh;
     __shared__ int sh_start[32], sh_pre[32];
+    // a halted step-graph replay leaves the step to the host
+    if (a.st->halt) return;
     const int slot = owner_slot((int)blockIdx.x, a.n);
     if (slot >= a.n) return;
     const int p = a.cell_idx[slot];
@@ -2107,6 +2113,16 @@ __global__ void k_selftest_fp64(long long n, const double *__restrict__ a, const
     sq[i] = __builtin_sqrt(a[i]);
     q[i] = a[i] / b[i];
     rs[i] = __builtin_amdgcn_rsq(a[i]);
+}
+
+// The in-range sequences of the bit-exact path, element-wise.
+__global__ void k_selftest_fp64_inrange(long long n, const double *__restrict__ a, const double *__restrict__ b,
+                                        double *__restrict__ sq, double *__restrict__ q)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sq[i] = sqrt_rn_inrange(a[i]);
+    q[i] = div_rn_inrange(a[i], b[i]);
 }
 
 GridDev grid_dev(const CellGrid &g)
@@ -2438,6 +2454,13 @@ void launch_selftest_fp64(hipStream_t s, long long n, const double *a, const dou
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_selftest_fp64, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, b, sq, q, rs);
+}
+
+void launch_selftest_fp64_inrange(hipStream_t s, long long n, const double *a, const double *b, double *sq,
+                                  double *q)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_selftest_fp64_inrange, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, b, sq, q);
 }
 
 }  // namespace tb3

@@ -227,6 +227,14 @@ class Engine:
         self._check(self._lib.tb3_selftest_fp64(self._ctx, a.size, _dp(a), _dp(b), _dp(sq), _dp(q), _dp(rs)))
         return sq, q, rs
 
+    def selftest_fp64_inrange(self, a, b):
+        """The bit-exact path's in-range sequences: device sqrt(a) and a / b."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        sq, q = np.zeros_like(a), np.zeros_like(a)
+        self._check(self._lib.tb3_selftest_fp64_inrange(self._ctx, a.size, _dp(a), _dp(b), _dp(sq), _dp(q)))
+        return sq, q
+
 
 # -- particle CSV (utility/utility.cpp:14-24, 54-68) ---------------------------
 
