@@ -165,6 +165,29 @@ tb3_status tb3_energy_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, double n
 tb3_status tb3_energy_grad_dev(tb3_ctx *ctx, int64_t n, const double *d_pos, double nu, double *d_E,
                                double *d_dEdx, void *stream, uint64_t *n_triplets);
 
+/* tb3_energy_grad_dev without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as tb3_energy_grad_dev (E and
+ * dE/dx of n positions in device memory, ordered on `stream`, complete on
+ * return), for the cutoff algorithm (P3BCA, cutoff > 0) in TB3_FP64 or
+ * TB3_FP32_MIXED on one particle system with a known periodic domain
+ * (tb3_configure's domain_L, or the domain of the particles last set);
+ * other configurations return TB3_EUNSUPPORTED or TB3_EINVAL.  The context
+ * supplies the device and those parameters only: the call calls no hipMalloc
+ * or hipFree and uses none of the context's device buffers.  Its working
+ * memory (cell list, staged positions, forces, status block) is a static
+ * device array of TB3_NOALLOC_SCRATCH_BYTES per GPU, held from the first
+ * context on that GPU until the process ends; a system whose working set
+ * does not fit (about 88 bytes per particle and 16 per cell, plus ~100 KB:
+ * some 350 000 particles at density 2 and cutoff 2) returns TB3_EINVAL.
+ * Every call on one GPU shares that array, which keeps the last call's
+ * contents; calls that overlap on one GPU (from several threads or
+ * contexts) race and must be serialised by the caller. */
+#define TB3_NOALLOC_SCRATCH_BYTES (32u << 20)
+tb3_status tb3_energy_grad_dev_noalloc(tb3_ctx *ctx, int64_t n, const double *d_pos, double nu, double *d_E,
+                                       double *d_dEdx, void *stream, uint64_t *n_triplets);
+
 /* Autotuner.  No counterpart in the reference.  For the cutoff algorithm,
  * times a force sweep of the current particles (on a copy; the state is
  * untouched) for each cell width / cutoff ratio in {1, 1.15, 1.3, 1.5, 1.75,

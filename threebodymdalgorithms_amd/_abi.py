@@ -61,6 +61,8 @@ SIGNATURES = {
                                       ctypes.c_void_p, _u64p]),
     "tb3_energy_grad_dev": (ctypes.c_int, [_ctxp, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double,
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _u64p]),
+    "tb3_energy_grad_dev_noalloc": (ctypes.c_int, [_ctxp, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double,
+                                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _u64p]),
     "tb3_autotune": (ctypes.c_int, [_ctxp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "tb3_set_profiling": (ctypes.c_int, [_ctxp, ctypes.c_int]),
     "tb3_get_profile": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64]),

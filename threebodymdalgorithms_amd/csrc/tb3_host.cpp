@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -87,6 +88,37 @@ struct Buffers {
 };
 
 constexpr int MAX_LIST = 16384;  // longest neighbour list the LDS staging accepts
+
+// Working memory of tb3_energy_grad_dev_noalloc: one static array per GPU,
+// loaded with the code object and held until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: its
+// device address comes from hipGetSymbolAddress, once per device (at
+// tb3_create, with that device current), and only that address is used.
+constexpr size_t NOALLOC_BYTES = TB3_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch_addr[MAX_DEVICES] = {};
+
+// Records this device's address of `scratch` (the device must be current).
+void init_scratch_address(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_addr[device]) return;
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Fail(TB3_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_addr[device] = static_cast<unsigned char *>(ptr);
+}
+
+unsigned char *scratch_address(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_addr[device];
+}
 
 }  // namespace
 
@@ -763,6 +795,7 @@ tb3_status tb3_create(int device, tb3_ctx **out)
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
         c->ncu = std::max(1, ncu);
         p3bca_dense_setup();
+        init_scratch_address(device);
         c->dense_grid = std::max(1, ncu) * p3bca_dense_blocks_per_cu();
         dev_alloc(c->d_slots, 1);
         HIPCHK(hipMemset(c->d_slots, 0, sizeof(Slots)));
@@ -1148,6 +1181,194 @@ tb3_status tb3_energy_grad_dev(tb3_ctx *c, int64_t n, const double *d_pos, doubl
                                double *d_dEdx, void *stream, uint64_t *n_triplets)
 {
     return dev_common(c, n, d_pos, nu, d_E, d_dEdx, stream, n_triplets, true);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Start state of a tb3_energy_grad_dev_noalloc call, written by a kernel
+// (no memset): a clear status block pointing at its slots, clear slots, and
+// the cell counters, per-cell work and scatter ticket at zero.
+__global__ void k_noalloc_clear(Status *st, Slots *slots, int *cell_cnt, unsigned *cell_work, unsigned *ticket,
+                                int ncell)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long *sw = reinterpret_cast<unsigned long long *>(slots);
+    const long long nsw = (long long)(sizeof(Slots) / sizeof(unsigned long long));
+    for (long long i = t; i < nsw; i += stride) sw[i] = 0ull;
+    for (long long i = t; i < ncell; i += stride) {
+        cell_cnt[i] = 0;
+        cell_work[i] = 0u;
+    }
+    if (t == 0) {
+        *ticket = 0u;
+        st->count[0] = st->count[1] = 0ull;
+        st->energy = 0.0;
+        for (int k = 0; k < 4; ++k) st->flags[k] = 0;
+        st->dqueue[0] = st->dqueue[1] = 0;
+        st->dense_n = 0;
+        st->want_dense = 0;
+        st->slots = slots;
+        st->halt = 0;
+        st->nstep = 0;
+        st->counts = nullptr;
+        st->nrep = 1;
+        st->rep_count = nullptr;
+        st->rep_energy = nullptr;
+    }
+}
+
+// Carves the working set of one call out of the scratch array: 256-byte
+// aligned pieces; returns the bytes needed (which may exceed the array).
+struct NoallocLayout {
+    Status *st;
+    Slots *slots;
+    DeviceState d;
+    CellGrid g;
+    int *owners;
+    long long *scan_tmp;
+};
+
+size_t noalloc_layout(unsigned char *base, long long n, int ncell, NoallocLayout *L)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t dn = (size_t)std::max<long long>(n, 1) * sizeof(double);
+    const size_t in = (size_t)std::max<long long>(n, 1) * sizeof(int);
+    const size_t cn = (size_t)std::max(ncell, 1) * sizeof(int);
+    NoallocLayout l;
+    l.st = reinterpret_cast<Status *>(take(sizeof(Status)));
+    l.slots = reinterpret_cast<Slots *>(take(sizeof(Slots)));
+    l.d.n = (int)n;
+    l.d.x = reinterpret_cast<double *>(take(dn));
+    l.d.y = reinterpret_cast<double *>(take(dn));
+    l.d.z = reinterpret_cast<double *>(take(dn));
+    l.d.fx = reinterpret_cast<double *>(take(dn));
+    l.d.fy = reinterpret_cast<double *>(take(dn));
+    l.d.fz = reinterpret_cast<double *>(take(dn));
+    l.g.cell_of = reinterpret_cast<int *>(take(in));
+    l.g.cell_rank = reinterpret_cast<int *>(take(in));
+    l.g.cell_idx = reinterpret_cast<int *>(take(in));
+    l.owners = reinterpret_cast<int *>(take(in));
+    l.g.xs = reinterpret_cast<double *>(take(dn));
+    l.g.ys = reinterpret_cast<double *>(take(dn));
+    l.g.zs = reinterpret_cast<double *>(take(dn));
+    l.g.cell_cnt = reinterpret_cast<int *>(take(cn));
+    l.g.cell_start = reinterpret_cast<int *>(take(cn + sizeof(int)));
+    l.g.order = reinterpret_cast<int *>(take(cn));
+    l.g.cell_work = reinterpret_cast<unsigned *>(take(cn));
+    l.g.scatter_ticket = reinterpret_cast<unsigned *>(take(sizeof(unsigned)));
+    l.scan_tmp = reinterpret_cast<long long *>(
+        take((size_t)scan_tmp_elems(std::max<long long>(n, ncell)) * sizeof(long long)));
+    if (L) *L = l;
+    return off;
+}
+
+}  // namespace
+
+extern "C" {
+
+tb3_status tb3_energy_grad_dev_noalloc(tb3_ctx *c, int64_t n, const double *d_pos, double nu, double *d_E,
+                                       double *d_dEdx, void *stream, uint64_t *n_triplets)
+{
+    if (!c) return TB3_EINVAL;
+    if (n < 0 || (n > 0 && (!d_pos || !d_dEdx)) || !d_E) return fail(c, TB3_EINVAL, "bad device pointers");
+    if (!std::isfinite(nu)) return fail(c, TB3_EINVAL, "nu must be finite");
+    if (!uses_cutoff(c))
+        return fail(c, TB3_EUNSUPPORTED, "the allocation-free call serves P3BCA with a positive cutoff");
+    if (c->precision == TB3_FP64_EXACT)
+        return fail(c, TB3_EUNSUPPORTED, "the allocation-free call serves the fp64 and fp32-mixed precisions");
+    if (c->nrep > 1) return fail(c, TB3_EUNSUPPORTED, "the allocation-free call serves one particle system");
+    if (!c->have_L)
+        return fail(c, TB3_EINVAL, "the allocation-free call needs the periodic domain: pass domain_L to "
+                                   "tb3_configure or set particles first");
+    return guarded(c, [&] {
+        unsigned char *base = scratch_address(c->device);
+        if (!base) throw Fail(TB3_EHIP, "the scratch array's device address was not recorded");
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        NoallocLayout l;
+        l.g.nc[0] = l.g.nc[1] = l.g.nc[2] = 1;
+        int nc[3];
+        grid_shape(c, c->L, nc);
+        const int ncell = nc[0] * nc[1] * nc[2];
+        const size_t need = noalloc_layout(base, n, ncell, &l);
+        if (need > NOALLOC_BYTES)
+            throw Fail(TB3_EINVAL, "the allocation-free call needs " + std::to_string(need) + " bytes of scratch "
+                                   "for " + std::to_string(n) + " particles in " + std::to_string(ncell) +
+                                   " cells; the scratch array holds " + std::to_string(NOALLOC_BYTES));
+        CellGrid &g = l.g;
+        for (int k = 0; k < 3; ++k) {
+            g.nc[k] = nc[k];
+            g.lo[k] = c->lo[k];
+            g.L[k] = c->L[k];
+        }
+        g.ncell = ncell;
+        g.nrep = 1;
+        DeviceState &d = l.d;
+        const int clear_blocks = (int)std::min<long long>(1024, (std::max<long long>(ncell, 12288) + 255) / 256);
+        hipLaunchKernelGGL(k_noalloc_clear, dim3(clear_blocks), dim3(256), 0, s, l.st, l.slots, g.cell_cnt,
+                           g.cell_work, g.scatter_ticket, ncell);
+        HIPCHK(hipGetLastError());
+        if (n >= 3) {
+            launch_soa_from_rows(s, (int)n, d_pos, d.x, d.y, d.z);
+            launch_cells_build(s, d, 0, 0.0, 0.0, 1, g, l.st, l.scan_tmp);
+        }
+        const CutoffSpec cs = cutoff_spec(c);
+        const int mixed = c->precision == TB3_FP32_MIXED ? 1 : 0;
+        // the overflow protocol of run_pass, with the context's device
+        // buffers replaced by the scratch array: the dense-owner kernel once
+        // an owner needs it, the per-owner kernel with a growing list
+        // capacity once a neighbourhood is too large for the cell kernel
+        bool cells = true, dense = false;
+        int cap = c->list_cap;
+        Status h;
+        for (int attempt = 0;; ++attempt) {
+            if (attempt > 0) {
+                launch_reset_status(s, l.st, 1);
+                launch_zero(s, d.fx, n);
+                launch_zero(s, d.fy, n);
+                launch_zero(s, d.fz, n);
+            }
+            if (n >= 3) {
+                if (cells) {
+                    launch_p3bca_cells(s, d, g, cs, nu, mixed, 1, l.st, 0, 1, l.owners, dense ? 1 : 0);
+                    if (dense)
+                        launch_p3bca_dense(s, d, g, cs, nu, mixed, 1, l.st, 0, 1, l.owners, c->dense_grid);
+                } else {
+                    launch_p3bca_fast(s, d, g, cs, nu, cap, mixed, 1, l.st, 0, 1);
+                }
+            }
+            launch_finalize(s, l.st);
+            HIPCHK(hipMemcpyAsync(&h, l.st, sizeof(Status), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (!h.flags[2]) break;
+            if (attempt >= 8) throw Fail(TB3_EHIP, "the cutoff sweep kept overflowing its LDS staging");
+            if (h.flags[3]) cells = false;
+            if (h.want_dense) dense = true;
+            if (!cells && h.flags[1] > cap) {
+                if (h.flags[1] > MAX_LIST)
+                    throw Fail(TB3_EUNSUPPORTED, "a particle has " + std::to_string(h.flags[1]) +
+                                                     " neighbours within the cutoff; at most " +
+                                                     std::to_string(MAX_LIST) + " are supported");
+                cap = std::min(next_pow2(h.flags[1] + h.flags[1] / 8), MAX_LIST);
+            }
+        }
+        HIPCHK(hipMemcpyAsync(d_E, &l.st->energy, sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (n > 0) {
+            if (n >= 3) launch_rows_from_soa(s, (int)n, d.fx, d.fy, d.fz, d_dEdx);
+            else launch_zero(s, d_dEdx, 3 * n);
+        }
+        HIPCHK(hipGetLastError());
+        // the scratch array is the next call's: the reads above complete first
+        HIPCHK(hipStreamSynchronize(s));
+        if (n_triplets) *n_triplets = h.count[0];
+    });
 }
 
 tb3_status tb3_autotune(tb3_ctx *c, int reps, double *best_ms, double *cell_ratio, int *blocks_per_cu)

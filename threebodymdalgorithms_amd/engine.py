@@ -200,6 +200,16 @@ class Engine:
                                                   ctypes.byref(nt) if want_count else None))
         return int(nt.value) if want_count else None
 
+    def energy_grad_dev_noalloc(self, n, d_pos, nu, d_E, d_grad, stream=None, want_count=False):
+        """tb3_energy_grad_dev without device allocation (the library's
+        static scratch array; P3BCA, fp64 or fp32-mixed)."""
+        nt = ctypes.c_uint64(0)
+        self._check(self._lib.tb3_energy_grad_dev_noalloc(self._ctx, int(n), ctypes.c_void_p(d_pos), float(nu),
+                                                          ctypes.c_void_p(d_E), ctypes.c_void_p(d_grad),
+                                                          ctypes.c_void_p(stream or 0),
+                                                          ctypes.byref(nt) if want_count else None))
+        return int(nt.value) if want_count else None
+
     def autotune(self, reps=3):
         """Pick the cell width / cutoff ratio and the cell kernel's resident
         workgroups per CU by timing force sweeps of the current particles."""
