@@ -173,6 +173,12 @@ struct tb3_ctx {
 
     // the replayed step graph of tb3_run, and what its capture baked in
     hipGraphExec_t step_exec = nullptr;
+    // cutoff grids that scan inside the scatter kernel replay the steps as
+    // one launch of the binning, then step_exec_mid (scatter, sweep, the
+    // update fused with the next step's binning) for all steps but the
+    // last, and step_exec (scatter, sweep, update) for the last
+    hipGraphExec_t step_exec_mid = nullptr;
+    bool graph_fused = false;
     unsigned long long epoch = 1;  // bumped whenever buffers, grid or parameters change
     unsigned long long graph_epoch = 0;
     double graph_dt = 0.0, graph_g[3] = {0, 0, 0};
@@ -609,36 +615,70 @@ bool graph_ok(const tb3_ctx *c)
     return c->cells_ok || p3bca_fast_lds(c->list_cap) <= 64 * 1024;
 }
 
-void capture_step_graph(tb3_ctx *c, double dt, const double *g)
+// The fused replay sequence serves the cutoff algorithm on grids whose
+// count scan runs inside the scatter kernel.
+bool fused_steps(const tb3_ctx *c) { return uses_cutoff(c) && c->state.d.n >= 3 && cells_fused_scan(c->grid); }
+
+template <class F>
+hipGraphExec_t capture(hipStream_t s, F &&body)
 {
-    if (c->step_exec && c->graph_epoch == c->epoch && c->graph_dt == dt && c->graph_g[0] == g[0] &&
-        c->graph_g[1] == g[1] && c->graph_g[2] == g[2] && c->graph_cap == c->list_cap &&
-        c->graph_cells_ok == c->cells_ok && c->graph_dense_on == c->dense_on)
-        return;
-    if (c->step_exec) {
-        (void)hipGraphExecDestroy(c->step_exec);
-        c->step_exec = nullptr;
-    }
-    hipStream_t s = c->stream;
-    DeviceState &d = c->state.d;
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-        launch_prepare(c, s, d, true, dt, true, true);
-        launch_sweep(c, s, d, true, false, false);
-        launch_tail(c, s, d, dt, g, true, true);
+        body();
     } catch (...) {
         (void)hipStreamEndCapture(s, &graph);
         if (graph) (void)hipGraphDestroy(graph);
         throw;
     }
     HIPCHK(hipStreamEndCapture(s, &graph));
-    const hipError_t e = hipGraphInstantiate(&c->step_exec, graph, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
+    HIPCHK(e);
+    return exec;
+}
+
+void capture_step_graph(tb3_ctx *c, double dt, const double *g)
+{
+    const bool fused = fused_steps(c);
+    if (c->step_exec && c->graph_epoch == c->epoch && c->graph_dt == dt && c->graph_g[0] == g[0] &&
+        c->graph_g[1] == g[1] && c->graph_g[2] == g[2] && c->graph_cap == c->list_cap &&
+        c->graph_cells_ok == c->cells_ok && c->graph_dense_on == c->dense_on && c->graph_fused == fused)
+        return;
+    if (c->step_exec) {
+        (void)hipGraphExecDestroy(c->step_exec);
         c->step_exec = nullptr;
-        HIPCHK(e);
     }
+    if (c->step_exec_mid) {
+        (void)hipGraphExecDestroy(c->step_exec_mid);
+        c->step_exec_mid = nullptr;
+    }
+
+    hipStream_t s = c->stream;
+    DeviceState &d = c->state.d;
+    if (fused) {
+        const bool exact = c->precision == TB3_FP64_EXACT;
+        auto mid = [&] {
+            launch_scatter_fused(s, d, c->grid, c->d_status);
+            launch_sweep(c, s, d, true, false, false);
+            launch_update_bin(s, d, dt * 0.5, g, c->d_status, exact ? 1 : 0, dt, dt * dt * 0.5, exact ? 0 : 1,
+                              c->grid);
+        };
+        c->step_exec_mid = capture(s, mid);
+        c->step_exec = capture(s, [&] {
+            launch_scatter_fused(s, d, c->grid, c->d_status);
+            launch_sweep(c, s, d, true, false, false);
+            launch_tail(c, s, d, dt, g, true, true);
+        });
+    } else {
+        c->step_exec = capture(s, [&] {
+            launch_prepare(c, s, d, true, dt, true, true);
+            launch_sweep(c, s, d, true, false, false);
+            launch_tail(c, s, d, dt, g, true, true);
+        });
+    }
+    c->graph_fused = fused;
     c->graph_epoch = c->epoch;
     c->graph_dt = dt;
     for (int k = 0; k < 3; ++k) c->graph_g[k] = g[k];
@@ -686,7 +726,16 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
         if (!uses_cutoff(c) && d.n >= 3 && c->precision != TB3_FP64_EXACT) ensure_triples(c, d.n);
         capture_step_graph(c, dt, g);
         launch_begin_steps(s, c->d_status, c->d_counts);
-        for (int k = done; k < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec, s));
+        if (c->graph_fused) {
+            // the first step's predictor and binning; each step_exec_mid
+            // replay ends with the next step's
+            launch_bin_ranked(s, d, 1, dt, dt * dt * 0.5, c->precision == TB3_FP64_EXACT ? 0 : 1, c->grid,
+                              c->d_status);
+            for (int k = done; k + 1 < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec_mid, s));
+            HIPCHK(hipGraphLaunch(c->step_exec, s));
+        } else {
+            for (int k = done; k < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec, s));
+        }
         // the status and every slot the replays could have recorded come back
         // behind one synchronisation
         HIPCHK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(Status), hipMemcpyDeviceToHost, s));
@@ -846,6 +895,7 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.rep_count);
     dev_free(c->grid.rep_energy);
     if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+    if (c->step_exec_mid) (void)hipGraphExecDestroy(c->step_exec_mid);
     for (hipEvent_t &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_status) (void)hipHostFree(c->h_status);

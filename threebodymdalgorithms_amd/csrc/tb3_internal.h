@@ -125,6 +125,18 @@ void launch_scan_i32(hipStream_t s, const int *in, int *out, int count, long lon
 // the exact sweep sorts each owner's list by index.
 void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double dt, double h, int zero_f,
                         CellGrid &g, Status *st, long long *tmp);
+// The two halves of launch_cells_build for grids that scan inside the
+// scatter kernel (cells_fused_scan): the binning (+ predictor, + force clear)
+// and the scan-and-scatter, which does nothing in a halted step-graph replay.
+bool cells_fused_scan(const CellGrid &g);
+void launch_bin_ranked(hipStream_t s, const DeviceState &d, int predict, double dt, double h, int zero_f,
+                       CellGrid &g, Status *st);
+void launch_scatter_fused(hipStream_t s, const DeviceState &d, CellGrid &g, Status *st);
+// A replayed step's end fused with the next step's start (cells_fused_scan
+// grids): launch_update_fin's work with record, then the next step's
+// predictor, force clear (zero_f) and binning.
+void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const double g3[3], Status *st,
+                       int sum_roles, double dt, double h, int zero_f, CellGrid &g);
 
 // Cutoff sweeps (P3BCA).  One wave per particle builds that particle's
 // neighbour list in LDS (capacity `cap`) from the cell list; a list longer

@@ -498,6 +498,83 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
 // 1/16-octave bins of the work (256 bins; ties in any order), one workgroup
 // per run: the calling workgroup sorts runs r0, r0 + rstep, ... below 8.
 // start(c) is cell c's first slot.
+// The end of one replayed step fused with the start of the next (cutoff
+// algorithm, grids of at most FUSED_SCAN_CELLS cells): k_update_fin's fold,
+// count record, status clear and Particle::Update (structs.hpp:57-99), then,
+// in the same thread, the next step's predictor (structs.hpp:40-55) on the
+// updated velocity and acceleration, the force clear and the binning of
+// k_bin<true>.  The same operations on the same values as the two kernels,
+// one launch fewer per step.  The clear keeps flags[0] (a non-finite
+// position, which the binning may raise concurrently): it can only be set
+// here when the last sweep ran without overflow, i.e. with every position
+// finite, or on the per-owner and exact kernels, where it is sticky anyway.
+__global__ __launch_bounds__(NSLOT) void k_update_bin(DeviceState d, double hdt, double gx, double gy, double gz,
+                                                     Status *st, int sum_roles, double dt, double h, int zero_f,
+                                                     GridDev g, int *__restrict__ cell_of,
+                                                     int *__restrict__ cell_cnt, int *__restrict__ cell_rank)
+{
+    if (blockIdx.x == 0) {
+        fold_slots(st);
+        __syncthreads();  // thread 0's fold (and halt) is visible to the workgroup
+        if (!st->halt) {
+            const int t = threadIdx.x, nrep = st->nrep, ns = st->nstep;
+            if (nrep > 1) {
+                for (int r = t; r < nrep; r += NSLOT) st->counts[(long long)ns * nrep + r] = st->rep_count[r];
+            } else if (t == 0) {
+                st->counts[ns] = st->count[0];
+            }
+            __syncthreads();  // every read of this step's sums precedes the clear
+            clear_status(st, true, t, NSLOT);
+            if (t == 0) st->nstep = ns + 1;
+        }
+    }
+    if (st->flags[2]) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.n) return;
+    double fx, fy, fz;
+    if (sum_roles) {
+        const long long n = d.n;
+        fx = d.facc[0 * n + p] + (d.facc[3 * n + p] + d.facc[6 * n + p]);
+        fy = d.facc[1 * n + p] + (d.facc[4 * n + p] + d.facc[7 * n + p]);
+        fz = d.facc[2 * n + p] + (d.facc[5 * n + p] + d.facc[8 * n + p]);
+        d.fx[p] = fx;
+        d.fy[p] = fy;
+        d.fz[p] = fz;
+    } else {
+        fx = d.fx[p];
+        fy = d.fy[p];
+        fz = d.fz[p];
+    }
+    const double m = d.m[p];
+    const double a0 = gx - fx / m, a1 = gy - fy / m, a2 = gz - fz / m;
+    const double vx = d.vx[p] + a0 * hdt, vy = d.vy[p] + a1 * hdt, vz = d.vz[p] + a2 * hdt;
+    d.vx[p] = vx;
+    d.vy[p] = vy;
+    d.vz[p] = vz;
+    d.ax[p] = a0;
+    d.ay[p] = a1;
+    d.az[p] = a2;
+    // the next step: predictor with aX for all three axes
+    const double px = predicted(d.x[p], vx, a0, dt, h);
+    const double py = predicted(d.y[p], vy, a0, dt, h);
+    const double pz = predicted(d.z[p], vz, a0, dt, h);
+    d.x[p] = px;
+    d.y[p] = py;
+    d.z[p] = pz;
+    if (zero_f) {
+        d.fx[p] = 0.0;
+        d.fy[p] = 0.0;
+        d.fz[p] = 0.0;
+    }
+    if (!finite3(px, py, pz)) atomicOr(&st->flags[0], 1);
+    const SysGrid sg = grid_of_particle(g, p);
+    int cx, cy, cz;
+    cell_of_pos(sg, px, py, pz, cx, cy, cz);
+    const int c = sg.cbase + (cz * sg.nc[1] + cy) * sg.nc[0] + cx;
+    cell_of[p] = c;
+    cell_rank[p] = atomicAdd(&cell_cnt[c], 1);
+}
+
 template <class Start>
 __device__ void order_runs(int ncell, const Start &start, const unsigned *__restrict__ cell_work,
                            int *__restrict__ order, int *hist, int r0, int rstep)
@@ -572,12 +649,16 @@ __global__ __launch_bounds__(256) void k_cell_scatter_fused(
     const int *__restrict__ cell_of, const int *__restrict__ cell_rank, int *__restrict__ cell_cnt,
     int *__restrict__ cell_start, int *__restrict__ cell_idx, double *__restrict__ xs, double *__restrict__ ys,
     double *__restrict__ zs, int ncell, int *__restrict__ order, const unsigned *__restrict__ cell_work,
-    unsigned *__restrict__ ticket, int nsc)
+    unsigned *__restrict__ ticket, int nsc, const Status *__restrict__ st)
 {
     __shared__ int lst[FUSED_SCAN_CELLS + 1];
     __shared__ int part[256];
     __shared__ int hist[256];
     __shared__ int s_last;
+    // A halted step-graph replay keeps the cell list of the step that
+    // overflowed (the host finishes that step on it); every workgroup reads
+    // the same halt, so the ticket stays consistent.
+    if (st->halt) return;
     const int t = threadIdx.x;
     const int p = blockIdx.x * blockDim.x + t;  // >= n in the ordering workgroups
     int pcell = 0, prank = 0;
@@ -2250,6 +2331,33 @@ void launch_scan_i32(hipStream_t s, const int *in, int *out, int count, long lon
     scan_rec<int, int>(s, in, count, out, (long long)count + 1, tmp);
 }
 
+bool cells_fused_scan(const CellGrid &g) { return g.ncell <= FUSED_SCAN_CELLS; }
+
+void launch_bin_ranked(hipStream_t s, const DeviceState &d, int predict, double dt, double h, int zero_f,
+                       CellGrid &g, Status *st)
+{
+    if (d.n <= 0) return;
+    hipLaunchKernelGGL(k_bin<true>, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, predict, dt, h, zero_f,
+                       grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank, st);
+}
+
+void launch_scatter_fused(hipStream_t s, const DeviceState &d, CellGrid &g, Status *st)
+{
+    if (d.n <= 0) return;
+    const int nsc = blocks_for(d.n, 256);
+    hipLaunchKernelGGL(k_cell_scatter_fused, dim3(nsc + 8), dim3(256), 0, s, d.n, d.x, d.y, d.z, g.cell_of,
+                       g.cell_rank, g.cell_cnt, g.cell_start, g.cell_idx, g.xs, g.ys, g.zs, g.ncell, g.order,
+                       g.cell_work, g.scatter_ticket, nsc, st);
+}
+
+void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const double g3[3], Status *st,
+                       int sum_roles, double dt, double h, int zero_f, CellGrid &g)
+{
+    const unsigned nb = d.n > 0 ? blocks_for(d.n, NSLOT) : 1;
+    hipLaunchKernelGGL(k_update_bin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g3[0], g3[1], g3[2], st, sum_roles, dt,
+                       h, zero_f, grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank);
+}
+
 void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double dt, double h, int zero_f,
                         CellGrid &g, Status *st, long long *tmp)
 {
@@ -2260,7 +2368,7 @@ void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double
         const int nsc = blocks_for(n, 256);
         hipLaunchKernelGGL(k_cell_scatter_fused, dim3(nsc + 8), dim3(256), 0, s, n, d.x, d.y, d.z, g.cell_of,
                            g.cell_rank, g.cell_cnt, g.cell_start, g.cell_idx, g.xs, g.ys, g.zs, g.ncell, g.order,
-                           g.cell_work, g.scatter_ticket, nsc);
+                           g.cell_work, g.scatter_ticket, nsc, st);
         return;
     }
     if (n > 0)
