@@ -41,15 +41,21 @@ def per_particle_rel(a, b):
     return np.max(num / np.where(den > 0, den, 1.0))
 
 
-def force_errors(f, fo):
+def force_errors(f, fo, scale=None):
     """Per-particle force errors of f against the reference fo: pure ratio
-    |df|/|F|, its RMS, and the ratio with |F| floored at the RMS force."""
+    |df|/|F|, its RMS, the ratio with |F| floored at the RMS force, and, given
+    each particle's summed contribution lengths S_p (oracle.force_scale),
+    max |df|/S_p: a sum of terms of total length S_p computed with relative
+    error e per term is off by at most e S_p, however much the terms cancel."""
     num = np.linalg.norm(f - fo, axis=1)
     den = np.linalg.norm(fo, axis=1)
     rel = num / np.where(den > 0, den, 1.0)
     floor = np.sqrt(np.mean(den ** 2))
-    return {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel ** 2))),
-            "max_rel_floored": float((num / np.maximum(den, floor)).max())}
+    out = {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel ** 2))),
+           "max_rel_floored": float((num / np.maximum(den, floor)).max())}
+    if scale is not None:
+        out["max_rel_scale"] = float((num / np.where(scale > 0, scale, 1.0)).max())
+    return out
 
 
 def gpu_forces(eng, P, algo, cutoff, precision, nu=1.0, L=None):
@@ -229,15 +235,20 @@ def test_fast_sweep_particles_outside_box(eng, oracle, precision):
     f, cnt = gpu_forces(eng, P, "p3bca", 2.0, precision, L=L)
     *_, fo, co = oracle.sweep(P, cutoff=2.0, L=L, mode="owner")
     assert cnt == co
-    e = force_errors(f, fo)
+    e = force_errors(f, fo, oracle.force_scale(P, cutoff=2.0, L=L))
     say(f"  errors {e}")
     # the moved particles' forces come from raw vectors about a box length
     # long and largely cancel, which inflates the pure ratio |df|/|F| of a
-    # few particles; the ratio floored at the RMS force carries the bar
+    # few particles; the ratio floored at the RMS force carries the FP64 bar.
+    # (The FP64 ratio to the summed contribution lengths is only printed: a
+    # triplet with a close pair hands its third particle two large, nearly
+    # opposite terms whose sum the reference's expression rounds to ~1e-10 of
+    # itself, so that ratio measures the reference's rounding there.)
     if precision == "fp64":
         assert e["max_rel_floored"] <= 1e-12 and e["rms_rel"] <= 1e-10
     else:
         assert e["max_rel_floored"] <= 1e-6 and e["rms_rel"] <= 1e-3
+        assert e["max_rel_scale"] <= 2e-6
 
 
 @pytest.mark.parametrize("algo,n,box,cutoff", [("auta", 600, 5.0, 0.0), ("p3bca", 4000, 12.6, 2.0),
@@ -249,16 +260,19 @@ def test_fp32_mixed_one_sweep(eng, oracle, algo, n, box, cutoff):
     f, cnt = gpu_forces(eng, P, algo, cutoff, "fp32")
     *_, fo, co = oracle.sweep(P, cutoff=cutoff if algo == "p3bca" else -1.0, L=L, mode="owner")
     assert cnt == co
-    e = force_errors(f, fo)
+    scale = oracle.force_scale(P, cutoff=cutoff if algo == "p3bca" else -1.0, L=L)
+    e = force_errors(f, fo, scale)
     say(f"  fp32 errors {e}")
-    # FP32 rounding of u, v, w (~6e-8 each) bounds each triplet to ~1e-7
-    # (close pairs included: their third particle's coefficient sum is formed
-    # without cancellation); particles whose net force nearly cancels amplify
-    # that in the pure per-particle ratio, so the floored ratio (floor = RMS
-    # |F|) carries the 1e-6 bar.
+    # FP32 rounding of u, v, w (~6e-8 each) bounds each triplet's contribution
+    # to ~1e-7 of its length, a few 1e-7 where a gradient coefficient is the
+    # difference of larger FP32 terms (close pairs included: their third
+    # particle's coefficient sum is formed without cancellation).  So the
+    # error is held against each particle's summed contribution lengths; the
+    # pure ratio |df|/|F| is unbounded where the contributions cancel and is
+    # only printed.
+    assert e["max_rel_scale"] <= 2e-6
     assert e["max_rel_floored"] <= 1e-6
     assert e["rms_rel"] <= 1e-5
-    assert e["max_rel"] <= 1e-3
 
 
 @pytest.mark.parametrize("precision", ["exact", "fp64", "fp32"])

@@ -75,7 +75,7 @@ __device__ __forceinline__ double sqr_dist_periodic_min(double ax, double ay, do
 // 2^-500 .. 2^500 those steps are the identity and sqrt_rn_inrange is the
 // iteration alone (10 instructions instead of 17).  Both are checked bit for
 // bit against the compiler's IEEE expansions on the device
-// (tb3_selftest_fp64_fast).
+// (tb3_selftest_fp64_inrange).
 __device__ __forceinline__ double div_rn_inrange(double a, double b)
 {
     double r = __builtin_amdgcn_rcp(b);
