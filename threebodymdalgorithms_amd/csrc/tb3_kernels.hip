@@ -1134,11 +1134,13 @@ __device__ __forceinline__ double cut_sq(double ax, double ay, double az, double
     }
 }
 
-// Global slot (cell order) of staged index t.
+// Global slot (cell order) of staged index t: cpre is nondecreasing with
+// cpre[0] = 0, so the staged cell holding t is the number of later cells
+// starting at or before t (independent LDS reads, no dependent chain).
 __device__ __forceinline__ int staged_slot(const int *cst, const int *cpre, int nk, int t)
 {
     int k = 0;
-    while (k + 1 < nk && cpre[k + 1] <= t) ++k;
+    for (int j = 1; j < nk; ++j) k += cpre[j] <= t ? 1 : 0;
     return cst[k] + (t - cpre[k]);
 }
 
@@ -1167,7 +1169,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     __shared__ double wls[CELL_WAVES][6][LCAP];
     __shared__ unsigned short ring[CELL_WAVES][RQ];
     __shared__ int cst[32], cpre[33], sinfo[4];
-    __shared__ int s_next, s_far, s_wcnt[CELL_WAVES];
+    __shared__ int s_next, s_far, s_wcnt[2][CELL_WAVES];
     __shared__ unsigned s_work;
 
     const CutDev &A = a.c;
@@ -1228,41 +1230,68 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
             const double g = __builtin_fabs(d) - 0.5 * w;
             return g > 0.0 ? g : 0.0;
         };
+        // two candidates per thread and round (a cell's neighbourhood has
+        // ~265 on average), with every load issued before the first use
         bool far = false;
-        for (int t0 = 0; t0 < cand; t0 += CELL_WAVES * WAVE) {
-            const int t = t0 + tid;
-            bool keep = false;
-            double x = 0.0, y = 0.0, z = 0.0;
-            int q = 0;
-            if (t < cand) {
-                const int slot = staged_slot(cst, cpre, nk, t);
-                x = a.xs[slot];
-                y = a.ys[slot];
-                z = a.zs[slot];
-                q = A.cell_idx[slot];
-                const double gx = gap(x - mx, Lx, wx), gy = gap(y - my, Ly, wy), gz = gap(z - mz, Lz, wz);
-                keep = t < own || gx * gx + gy * gy + gz * gz <= reach2;
+        constexpr int NT = CELL_WAVES * WAVE;
+        for (int t0 = 0; t0 < cand; t0 += 2 * NT) {
+            bool keep[2];
+            double x[2], y[2], z[2];
+            int q[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int t = t0 + h * NT + tid;
+                keep[h] = false;
+                x[h] = y[h] = z[h] = 0.0;
+                q[h] = 0;
+                if (t < cand) {
+                    const int slot = staged_slot(cst, cpre, nk, t);
+                    x[h] = a.xs[slot];
+                    y[h] = a.ys[slot];
+                    z[h] = a.zs[slot];
+                    q[h] = A.cell_idx[slot];
+                }
             }
-            const unsigned long long m = __ballot(keep);
-            if (lane == 0) s_wcnt[wv] = __popcll(m);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int t = t0 + h * NT + tid;
+                if (t < cand) {
+                    const double gx = gap(x[h] - mx, Lx, wx), gy = gap(y[h] - my, Ly, wy),
+                                 gz = gap(z[h] - mz, Lz, wz);
+                    keep[h] = t < own || gx * gx + gy * gy + gz * gz <= reach2;
+                }
+            }
+            const unsigned long long m0 = __ballot(keep[0]), m1 = __ballot(keep[1]);
+            if (lane == 0) {
+                s_wcnt[0][wv] = __popcll(m0);
+                s_wcnt[1][wv] = __popcll(m1);
+            }
             __syncthreads();
-            int pos = total + lane_prefix(m);
-            for (int w = 0; w < CELL_WAVES; ++w) {
-                const int k = s_wcnt[w];
-                if (w < wv) pos += k;
-                total += k;
+            // the kept candidates of the first half, then of the second, in order
+            int pos[2] = {total + lane_prefix(m0), lane_prefix(m1)};
+            for (int h = 0; h < 2; ++h) {
+                if (h == 1) pos[1] += total;
+                for (int w = 0; w < CELL_WAVES; ++w) {
+                    const int k = s_wcnt[h][w];
+                    if (w < wv) pos[h] += k;
+                    total += k;
+                }
             }
-            if (keep && pos < NBCAP) {
-                nx[pos] = x;
-                ny[pos] = y;
-                nz[pos] = z;
-                nid[pos] = q;
-                gxs[pos] = 0.0;
-                gys[pos] = 0.0;
-                gzs[pos] = 0.0;
-                const double ux = (x - sg.lo[0]) * ix - ccx, uy = (y - sg.lo[1]) * iy - ccy,
-                             uz = (z - sg.lo[2]) * iz - ccz;
-                far = far || !(ux >= -1.01 && ux < 2.01 && uy >= -1.01 && uy < 2.01 && uz >= -1.01 && uz < 2.01);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (keep[h] && pos[h] < NBCAP) {
+                    const int p = pos[h];
+                    nx[p] = x[h];
+                    ny[p] = y[h];
+                    nz[p] = z[h];
+                    nid[p] = q[h];
+                    gxs[p] = 0.0;
+                    gys[p] = 0.0;
+                    gzs[p] = 0.0;
+                    const double ux = (x[h] - sg.lo[0]) * ix - ccx, uy = (y[h] - sg.lo[1]) * iy - ccy,
+                                 uz = (z[h] - sg.lo[2]) * iz - ccz;
+                    far = far || !(ux >= -1.01 && ux < 2.01 && uy >= -1.01 && uy < 2.01 && uz >= -1.01 && uz < 2.01);
+                }
             }
             __syncthreads();  // s_wcnt is reused
         }
