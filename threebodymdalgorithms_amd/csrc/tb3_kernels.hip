@@ -105,10 +105,15 @@ __device__ __forceinline__ void add_energy(Status *st, double e)
     if (e != 0.0) unsafeAtomicAdd(&st->slots->energy[(blockIdx.x & (NSLOT - 1)) * SLOT_STRIDE], e);
 }
 
-// The longest neighbour list matters only near or past the capacity.
+// The longest neighbour list matters only past the capacity (the host then
+// grows the capacity to fit it).  Atomics on one address from every owner
+// serialise in the memory system (tens of thousands of them cost
+// milliseconds), so only lists over the capacity, and longer than the value
+// already recorded, issue one.
 __device__ __forceinline__ void note_list_length(Status *st, int len, int cap)
 {
-    if (2 * len > cap) atomicMax(&st->flags[1], len);
+    if (len > cap && len > __hip_atomic_load(&st->flags[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&st->flags[1], len);
 }
 
 // ---------------------------------------------------------------------------
@@ -1882,8 +1887,8 @@ struct StagedList {
 
 template <int ROLE>
 __device__ void exact_role_staged(const StagedList &L, double xp, double yp, double zp, int len, int split,
-                                  double c2, double Lx, double Ly, double Lz, double nu, ExactShared &sh, double &acc,
-                                  unsigned long long &accepted)
+                                  bool plain, double c2, double Lx, double Ly, double Lz, double nu, ExactShared &sh,
+                                  double &acc, unsigned long long &accepted)
 {
     const int lane = lane_id();
     const bool nu_ok = nu_inrange(nu);
@@ -1902,7 +1907,10 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
         m = split;
         m2 = 0;
     }
-    const long long nitems = ROLE == 1 ? (long long)m * m2 : (long long)m * (m - 1) / 2;
+    // len <= 2048 (the list and its positions fill at most 64 KB of LDS), so
+    // item numbers, row starts and (2m - 1)^2 stay below 2^24: exact in int
+    // and in FP32
+    const int nitems = ROLE == 1 ? m * m2 : m * (m - 1) / 2;
     acc = 0.0;  // lanes 0, 1, 2 hold x, y, z
     int head = 0, qn = 0;
 
@@ -1933,16 +1941,18 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
         qn -= cnt;
     };
 
-    const double twom1 = (double)(2 * m - 1);
-    const double inv_m2 = m2 > 0 ? 1.0 / (double)m2 : 0.0;
-    for (long long t0 = 0; t0 < nitems; t0 += WAVE) {
-        const long long t = t0 + lane;
+    const float twom1 = (float)(2 * m - 1), disc0 = (float)((2 * m - 1) * (2 * m - 1));
+    const float inv_m2 = m2 > 0 ? 1.0f / (float)m2 : 0.0f;
+    for (int t0 = 0; t0 < nitems; t0 += WAVE) {
+        const int t = t0 + lane;
         bool ok = t < nitems;
         int ia = 0, ib = 0;
         if (ok) {
+            // the row from an FP32 estimate (within one of the true row),
+            // then corrected with exact integer comparisons
             if constexpr (ROLE == 1) {
-                int ra = (int)((double)t * inv_m2);
-                long long rem = t - (long long)ra * m2;
+                int ra = (int)((float)t * inv_m2);
+                int rem = t - ra * m2;
                 while (rem < 0) {
                     --ra;
                     rem += m2;
@@ -1952,21 +1962,30 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
                     rem -= m2;
                 }
                 ia = ra;
-                ib = split + (int)rem;
+                ib = split + rem;
             } else {
                 // largest row a with a (2m - a - 1) / 2 <= t
-                const double disc = twom1 * twom1 - 8.0 * (double)t;
-                int ra = (int)((twom1 - __builtin_sqrt(disc > 0.0 ? disc : 0.0)) * 0.5);
-                if (ra < 0) ra = 0;
-                if (ra > m - 2) ra = m - 2;
-                auto start = [&](int r) { return (long long)r * (2 * m - r - 1) / 2; };
+                const float disc = disc0 - 8.0f * (float)t;
+                int ra = (int)((twom1 - __builtin_amdgcn_sqrtf(disc)) * 0.5f);
+                ra = ra < 0 ? 0 : (ra > m - 2 ? m - 2 : ra);
+                auto start = [&](int r) { return r * (2 * m - r - 1) / 2; };
                 while (ra > 0 && start(ra) > t) --ra;
                 while (ra < m - 2 && start(ra + 1) <= t) ++ra;
                 ia = s0 + ra;
-                ib = ia + 1 + (int)(t - start(ra));
+                ib = ia + 1 + (t - start(ra));
             }
-            ok = !(sqr_dist_periodic_min(L.px[ia], L.py[ia], L.pz[ia], L.px[ib], L.py[ib], L.pz[ib], Lx, Ly, Lz) >
-                   c2);
+            const double ax = L.px[ia], ay = L.py[ia], az = L.pz[ia];
+            const double bx = L.px[ib], by = L.py[ib], bz = L.pz[ib];
+            double d2;
+            if (plain) {
+                // the periodic form's bits whenever every |difference| is
+                // below L/2 (min(t, L - t) is then t, and t * t = d * d)
+                const double dx = bx - ax, dy = by - ay, dz = bz - az;
+                d2 = dx * dx + dy * dy + dz * dz;
+            } else {
+                d2 = sqr_dist_periodic_min(ax, ay, az, bx, by, bz, Lx, Ly, Lz);
+            }
+            ok = !(d2 > c2);
         }
         const unsigned long long mk = __ballot(ok);
         if (ok) {
@@ -2046,22 +2065,32 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_exact(CutDev a)
     for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o);
     const int split = below;
 
-    // stage the sorted list's positions next to it
+    // stage the sorted list's positions next to it.  When every neighbour
+    // lies within 0.24 L of p in every dimension, any two of them differ by
+    // less than L / 2, and the pair tests may take the plain form.
     double *spx = reinterpret_cast<double *>(sid_ex + cap);
     double *spy = spx + cap, *spz = spy + cap;
+    bool far = false;
     for (int t = lane; t < len; t += WAVE) {
         const int q = sid_ex[t];
-        spx[t] = a.x[q];
-        spy[t] = a.y[q];
-        spz[t] = a.z[q];
+        const double qx = a.x[q], qy = a.y[q], qz = a.z[q];
+        spx[t] = qx;
+        spy[t] = qy;
+        spz[t] = qz;
+        far = far || !(__builtin_fabs(qx - xp) <= 0.24 * sg.L[0] && __builtin_fabs(qy - yp) <= 0.24 * sg.L[1] &&
+                       __builtin_fabs(qz - zp) <= 0.24 * sg.L[2]);
     }
+    const bool plain = __ballot(far) == 0;
     __syncthreads();
     const StagedList list{spx, spy, spz};
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     unsigned long long accepted = 0, unused = 0;
-    exact_role_staged<0>(list, xp, yp, zp, len, split, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc0, accepted);
-    exact_role_staged<1>(list, xp, yp, zp, len, split, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc1, unused);
-    exact_role_staged<2>(list, xp, yp, zp, len, split, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc2, unused);
+    exact_role_staged<0>(list, xp, yp, zp, len, split, plain, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc0,
+                         accepted);
+    exact_role_staged<1>(list, xp, yp, zp, len, split, plain, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc1,
+                         unused);
+    exact_role_staged<2>(list, xp, yp, zp, len, split, plain, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc2,
+                         unused);
     if (lane < 3) {
         a.facc[(long long)lane * a.n + p] = acc0;
         a.facc[(long long)(3 + lane) * a.n + p] = acc1;
