@@ -70,7 +70,7 @@ struct Status {
     unsigned long long count[2];  // [0] accepted triplets, [1] discarded count of an auxiliary pass
     double energy;
     int flags[4];                 // [0] non-finite position seen, [1] longest neighbour list (recorded
-                                  // when above half the capacity), [2] list overflow (the update
+                                  // when above the capacity), [2] list overflow (the update
                                   // kernel skips itself), [3] the cell kernel could not stage a
                                   // neighbourhood (re-run with the per-owner kernel)
     int dqueue[2];                // owner queue of the dense-owner kernel; returns to zero after each launch
