@@ -1883,7 +1883,7 @@ __global__ __launch_bounds__(WAVE) void k_auta_exact(AutaExactDev a)
 // pairs enter the queue in item order, which is the stream order, so the
 // fold below adds the contributions exactly as the nested loops did.
 struct StagedList {
-    const double *px, *py, *pz;
+    const double *p;  // entry t at p[3t], p[3t + 1], p[3t + 2] (x, y, z side by side: two LDS reads)
 };
 
 template <int ROLE>
@@ -1919,9 +1919,9 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
         __syncthreads();
         if (lane < cnt) {
             const int q = (head + lane) & (RING - 1);
-            const int ia = sh.qa[q], ib = sh.qb[q];
-            const double xa = L.px[ia], ya = L.py[ia], za = L.pz[ia];
-            const double xb = L.px[ib], yb = L.py[ib], zb = L.pz[ib];
+            const int ia = sh.qa[q] & 0xffff, ib = sh.qa[q] >> 16;
+            const double xa = L.p[3 * ia], ya = L.p[3 * ia + 1], za = L.p[3 * ia + 2];
+            const double xb = L.p[3 * ib], yb = L.p[3 * ib + 1], zb = L.p[3 * ib + 2];
             double cx, cy, cz;
             if constexpr (ROLE == 0)
                 ref_contrib<0>(xp, yp, zp, xa, ya, za, xb, yb, zb, nu, nu_ok, cx, cy, cz);
@@ -1983,8 +1983,8 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
                 ia = s0 + ra;
                 ib = ia + 1 + (t - st);
             }
-            const double ax = L.px[ia], ay = L.py[ia], az = L.pz[ia];
-            const double bx = L.px[ib], by = L.py[ib], bz = L.pz[ib];
+            const double ax = L.p[3 * ia], ay = L.p[3 * ia + 1], az = L.p[3 * ia + 2];
+            const double bx = L.p[3 * ib], by = L.p[3 * ib + 1], bz = L.p[3 * ib + 2];
             double d2;
             if (plain) {
                 // the periodic form's bits whenever every |difference| is
@@ -1999,8 +1999,7 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
         const unsigned long long mk = __ballot(ok);
         if (ok) {
             const int q = (head + qn + lane_prefix(mk)) & (RING - 1);
-            sh.qa[q] = ia;
-            sh.qb[q] = ib;
+            sh.qa[q] = ia | (ib << 16);  // list indices are below 2048
         }
         const int got = __popcll(mk);
         qn += got;
@@ -2038,8 +2037,7 @@ __global__ __launch_bounds__(WAVE, 5) void k_p3bca_exact(CutDev a)
     const int cap = a.cap;
     const double xp = a.xs[slot], yp = a.ys[slot], zp = a.zs[slot];
     const SysGrid sg = grid_of_particle(a.g, p);
-    double *spx = reinterpret_cast<double *>(sid_ex + cap);
-    double *spy = spx + cap, *spz = spy + cap;
+    double *sp = reinterpret_cast<double *>(sid_ex + cap);  // 3 cap doubles: x, y, z of entry t at 3t
 
     const Candidates cand = gather_candidates(a, sg, xp, yp, zp, sh_start, sh_pre);
     int len = 0;
@@ -2079,9 +2077,9 @@ __global__ __launch_bounds__(WAVE, 5) void k_p3bca_exact(CutDev a)
             const int pos = len + lane_prefix(m);
             if (ok && pos < cap) {
                 sid_ex[pos] = q[h];
-                spx[pos] = qx[h];
-                spy[pos] = qy[h];
-                spz[pos] = qz[h];
+                sp[3 * pos] = qx[h];
+                sp[3 * pos + 1] = qy[h];
+                sp[3 * pos + 2] = qz[h];
             }
             len += __popcll(m);
         }
@@ -2118,9 +2116,9 @@ __global__ __launch_bounds__(WAVE, 5) void k_p3bca_exact(CutDev a)
             ex[h] = ey[h] = ez[h] = 0.0;
             if (t < len) {
                 id[h] = sid_ex[t];
-                ex[h] = spx[t];
-                ey[h] = spy[t];
-                ez[h] = spz[t];
+                ex[h] = sp[3 * t];
+                ey[h] = sp[3 * t + 1];
+                ez[h] = sp[3 * t + 2];
                 note_far(ex[h], ey[h], ez[h]);
                 below += id[h] < p ? 1 : 0;
             }
@@ -2137,9 +2135,9 @@ __global__ __launch_bounds__(WAVE, 5) void k_p3bca_exact(CutDev a)
         for (int h = 0; h < EXACT_GATHER; ++h) {
             if (h * WAVE + lane < len) {
                 sid_ex[rk[h]] = id[h];
-                spx[rk[h]] = ex[h];
-                spy[rk[h]] = ey[h];
-                spz[rk[h]] = ez[h];
+                sp[3 * rk[h]] = ex[h];
+                sp[3 * rk[h] + 1] = ey[h];
+                sp[3 * rk[h] + 2] = ez[h];
             }
         }
     } else {
@@ -2168,9 +2166,9 @@ __global__ __launch_bounds__(WAVE, 5) void k_p3bca_exact(CutDev a)
             const int q = sid_ex[t];
             below += q < p ? 1 : 0;
             const double qx = a.x[q], qy = a.y[q], qz = a.z[q];
-            spx[t] = qx;
-            spy[t] = qy;
-            spz[t] = qz;
+            sp[3 * t] = qx;
+            sp[3 * t + 1] = qy;
+            sp[3 * t + 2] = qz;
             note_far(qx, qy, qz);
         }
     }
@@ -2179,7 +2177,7 @@ __global__ __launch_bounds__(WAVE, 5) void k_p3bca_exact(CutDev a)
     const int split = below;
     const bool plain = __ballot(far) == 0;
     __syncthreads();
-    const StagedList list{spx, spy, spz};
+    const StagedList list{sp};
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     unsigned long long accepted = 0, unused = 0;
     exact_role_staged<0>(list, xp, yp, zp, len, split, plain, a.c2, sg.L[0], sg.L[1], sg.L[2], a.nu, sh, acc0,
