@@ -191,12 +191,24 @@ tb3_status tb3_energy_grad_dev_noalloc(tb3_ctx *ctx, int64_t n, const double *d_
 /* Autotuner.  No counterpart in the reference.  For the cutoff algorithm,
  * times a force sweep of the current particles (on a copy; the state is
  * untouched) for each cell width / cutoff ratio in {1, 1.15, 1.3, 1.5, 1.75,
- * 2}, keeps the fastest, and reports it (each output may be NULL): the best
- * sweep time in ms (HIP events, minimum of `reps` runs after one warm-up),
- * the ratio, and the cell kernel's resident workgroups per CU (0 when the
- * per-owner kernel is in charge).  The all-triplet sweeps have nothing to
- * tune and return at once. */
+ * 2} and keeps the fastest; then, when the dense-owner kernel runs (owners
+ * whose neighbourhood outgrows the cell kernel's staging), times it with 4, 8
+ * and 16 waves per owner and keeps the fastest.  Reports (each output may be
+ * NULL) the best sweep time in ms (HIP events, minimum of `reps` runs after
+ * one warm-up), the ratio, and the cell kernel's resident workgroups per CU
+ * (0 when the per-owner kernel is in charge); tb3_get_tuning reports every
+ * knob.  The all-triplet sweeps have nothing to tune and return at once. */
 tb3_status tb3_autotune(tb3_ctx *ctx, int reps, double *best_ms, double *cell_ratio, int *blocks_per_cu);
+
+/* The tuned knobs (each output may be NULL): cell width / cutoff ratio, the
+ * cell kernel's resident workgroups per CU, and the dense-owner kernel's
+ * waves per owner.  No counterpart in the reference. */
+tb3_status tb3_get_tuning(const tb3_ctx *ctx, double *cell_ratio, int *blocks_per_cu, int *dense_waves);
+
+/* Sets the knobs directly (to reuse a tuning, or to test each setting):
+ * cell_ratio in [1, 4], dense_waves 4, 8 or 16; TB3_EINVAL otherwise.  No
+ * counterpart in the reference. */
+tb3_status tb3_set_tuning(tb3_ctx *ctx, double cell_ratio, int dense_waves);
 
 /* Per-phase timing with HIP events recorded on the context's stream around
  * each phase of tb3_step, tb3_run (which runs its steps one pass at a time

@@ -189,6 +189,60 @@ def test_autotune_keeps_results(eng):
     assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max() <= 1e-12
 
 
+def dense_core_system():
+    """Uniform density-2 particles around a Gaussian core whose owners
+    outgrow the cell kernel's staging (they go to the dense-owner kernel)."""
+    rng = np.random.default_rng(17)
+    P = np.concatenate([uniform_particles(4000, 12.6, 21), rng.normal(6.3, 0.9, (400, 3))])
+    return P
+
+
+@pytest.mark.parametrize("waves", [4, 8, 16])
+def test_dense_waves_same_forces(eng, oracle, waves):
+    """Each workgroup size of the dense-owner kernel gives the reference's
+    counts and forces within the FP64 bar, from one step and from a replayed
+    run (whose captured graph must pick up the new size)."""
+    P = dense_core_system()
+    L, _ = oracle.domain_size(P)
+    eng.set_particles(P)
+    eng.configure("p3bca", cutoff=2.0, precision="fp64", domain_L=L)
+    eng.set_tuning(1.0, waves)
+    assert eng.get_tuning()["dense_waves"] == waves
+    say(f"dense waves {waves}: p3bca n={len(P)}")
+    cnt = eng.compute_forces()
+    f = eng.get_particles()["f"]
+    *_, fo, co = oracle.sweep(P, cutoff=2.0, L=L, mode="owner")
+    assert cnt == co
+    den = np.linalg.norm(fo, axis=1)
+    num = np.linalg.norm(f - fo, axis=1)
+    assert (num / np.maximum(den, np.sqrt(np.mean(den ** 2)))).max() <= 1e-12
+    # a replayed step after a switch of the size: same counts as stepping
+    V = np.zeros_like(P)
+    eng.set_particles(P, V, V, np.ones(len(P)))
+    c_run = eng.run(2, 0.001, (0.0, -9.81, 0.0))
+    eng.set_tuning(1.0, 8 if waves != 8 else 4)
+    eng.set_particles(P, V, V, np.ones(len(P)))
+    c_other = eng.run(2, 0.001, (0.0, -9.81, 0.0))
+    assert [int(x) for x in c_run][:1] == [int(x) for x in c_other][:1] == [co]
+
+
+def test_autotune_dense_waves(eng):
+    """Where the dense-owner kernel runs, the tuner also picks its waves per
+    owner; the counts stay the same."""
+    P = dense_core_system()
+    eng.set_particles(P)
+    eng.configure("p3bca", cutoff=2.0, precision="fp64")
+    c0 = eng.compute_forces()
+    t = eng.autotune(reps=1)
+    say(f"  tuned {t}")
+    assert t["dense_waves"] in (4, 8, 16) and t["sweep_ms"] > 0
+    assert eng.compute_forces() == c0
+    with pytest.raises(tb3.TB3Error):
+        eng.set_tuning(1.0, 12)
+    with pytest.raises(tb3.TB3Error):
+        eng.set_tuning(0.5, 8)
+
+
 def test_run_after_new_particles(eng):
     """A new particle set in an unchanged grid (fixed domain, same count)
     reuses the captured step graph; another particle count must capture

@@ -208,6 +208,8 @@ def batch256(skip_exact=False, steps=100):
             eng.set_particles(P)
             eng.configure("p3bca", cutoff=2.0, precision=prec)
             eng.batch_set(off)
+            # the cell width and the dense-owner kernel's waves per owner
+            tuned = eng.autotune(reps=2)
             eng.batch_run(2, 0.001, g)
             # the force sweep of the initial state (the densest), then K steps
             eng.set_particles(P)
@@ -224,7 +226,7 @@ def batch256(skip_exact=False, steps=100):
             total = int(counts.sum())
             out[prec] = {"steps": steps, "ms_per_step": secs / steps * 1e3, "triplets_per_step": total / steps,
                          "triplets_per_s": total / secs, "step0_sweep_ms": sweep_ms, "step0_triplets": c0,
-                         "step0_sweep_triplets_per_s": c0 / (sweep_ms * 1e-3)}
+                         "step0_sweep_triplets_per_s": c0 / (sweep_ms * 1e-3), "tuned": tuned}
         out["fp32_speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"]
         out["fp32_sweep_speedup_vs_fp64"] = out["fp64"]["step0_sweep_ms"] / out["fp32"]["step0_sweep_ms"]
         if not skip_exact:

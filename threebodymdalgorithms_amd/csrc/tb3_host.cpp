@@ -155,7 +155,8 @@ struct tb3_ctx {
     int list_cap = 128;    // LDS neighbour-list capacity per wave (power of two)
     bool cells_ok = true;  // the cell-per-workgroup kernel staged every neighbourhood so far
     bool dense_on = false;  // the dense-owner kernel follows the cell kernel (once some owner needed it)
-    int dense_grid = 1;    // persistent grid of the dense-owner kernel
+    int dense_grid = 1;    // persistent grid of the dense-owner kernel (workgroups that fit the GPU)
+    int dense_waves = 8;   // waves per owner in the dense-owner kernel (tuned: 4, 8 or 16)
     int ncu = 1;           // compute units of the device
     // tuned by tb3_autotune: cell width / cutoff, resident cell-kernel workgroups per CU
     double cell_ratio = 1.0;
@@ -466,7 +467,7 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
                                    energy ? 1 : 0, c->d_owners, c->dense_on ? 1 : 0);
                 if (c->dense_on)
                     launch_p3bca_dense(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
-                                       energy ? 1 : 0, c->d_owners, c->dense_grid);
+                                       energy ? 1 : 0, c->d_owners, c->dense_grid, c->dense_waves);
             }
             else if (cut)
                 launch_p3bca_fast(s, d, c->grid, cs, c->nu, c->list_cap, mixed, fast_grad ? 1 : 0, c->d_status,
@@ -845,7 +846,7 @@ tb3_status tb3_create(int device, tb3_ctx **out)
         c->ncu = std::max(1, ncu);
         p3bca_dense_setup();
         init_scratch_address(device);
-        c->dense_grid = std::max(1, ncu) * p3bca_dense_blocks_per_cu();
+        c->dense_grid = std::max(1, ncu) * p3bca_dense_blocks_per_cu(c->dense_waves);
         dev_alloc(c->d_slots, 1);
         HIPCHK(hipMemset(c->d_slots, 0, sizeof(Slots)));
         dev_alloc(c->d_status, 1);
@@ -1389,7 +1390,8 @@ tb3_status tb3_energy_grad_dev_noalloc(tb3_ctx *c, int64_t n, const double *d_po
                 if (cells) {
                     launch_p3bca_cells(s, d, g, cs, nu, mixed, 1, l.st, 0, 1, l.owners, dense ? 1 : 0);
                     if (dense)
-                        launch_p3bca_dense(s, d, g, cs, nu, mixed, 1, l.st, 0, 1, l.owners, c->dense_grid);
+                        launch_p3bca_dense(s, d, g, cs, nu, mixed, 1, l.st, 0, 1, l.owners, c->dense_grid,
+                                           c->dense_waves);
                 } else {
                     launch_p3bca_fast(s, d, g, cs, nu, cap, mixed, 1, l.st, 0, 1);
                 }
@@ -1421,6 +1423,19 @@ tb3_status tb3_energy_grad_dev_noalloc(tb3_ctx *c, int64_t n, const double *d_po
     });
 }
 
+namespace {
+
+// Waves per owner of the dense-owner kernel, and its persistent grid.
+void set_dense_waves(tb3_ctx *c, int waves)
+{
+    if (waves == c->dense_waves) return;
+    c->dense_waves = waves;
+    c->dense_grid = c->ncu * p3bca_dense_blocks_per_cu(waves);
+    ++c->epoch;  // captured step graphs hold the old launch
+}
+
+}  // namespace
+
 tb3_status tb3_autotune(tb3_ctx *c, int reps, double *best_ms, double *cell_ratio, int *blocks_per_cu)
 {
     if (!c) return TB3_EINVAL;
@@ -1450,49 +1465,93 @@ tb3_status tb3_autotune(tb3_ctx *c, int reps, double *best_ms, double *cell_rati
                 (void)hipEventDestroy(b);
             }
         } guard{e0, e1};
+        const bool was_prof = c->prof;
+        c->prof = false;
+        // best of `reps` force sweeps of the current particles on a scratch copy
+        auto time_sweep = [&]() {
+            double t = 1e300;
+            for (int r = 0; r < reps + 1; ++r) {  // the first run warms up
+                if (c->n > 0) {
+                    const size_t b = (size_t)c->n * sizeof(double);
+                    HIPCHK(hipMemcpyAsync(sd.x, d.x, b, hipMemcpyDeviceToDevice, s));
+                    HIPCHK(hipMemcpyAsync(sd.y, d.y, b, hipMemcpyDeviceToDevice, s));
+                    HIPCHK(hipMemcpyAsync(sd.z, d.z, b, hipMemcpyDeviceToDevice, s));
+                }
+                HIPCHK(hipEventRecord(e0, s));
+                run_pass(c, s, sd, false, 0.0, nullptr, true, false, false);
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) t = std::min(t, (double)ms);
+            }
+            return t;
+        };
+        // 1. the cell width: the cell kernel runs one workgroup per cell, so
+        // the width is its one free parameter (the hardware keeps
+        // p3bca_cells_blocks_per_cu of them resident on each CU)
         const int max_per_cu = p3bca_cells_blocks_per_cu();
         const double ratios[] = {1.0, 1.15, 1.3, 1.5, 1.75, 2.0};
         double best = 1e300, best_ratio = c->cell_ratio;
         int best_blocks = c->cells_per_cu;
-        const bool was_prof = c->prof;
-        c->prof = false;
         for (double ratio : ratios) {
             c->cell_ratio = ratio;
             configure_grid(c);
-            {
-                // the cell kernel runs one workgroup per cell; the hardware
-                // keeps max_per_cu of them resident on each CU
-                const int per_cu = max_per_cu;
-                double t = 1e300;
-                for (int r = 0; r < reps + 1; ++r) {  // the first run warms up
-                    if (c->n > 0) {
-                        const size_t b = (size_t)c->n * sizeof(double);
-                        HIPCHK(hipMemcpyAsync(sd.x, d.x, b, hipMemcpyDeviceToDevice, s));
-                        HIPCHK(hipMemcpyAsync(sd.y, d.y, b, hipMemcpyDeviceToDevice, s));
-                        HIPCHK(hipMemcpyAsync(sd.z, d.z, b, hipMemcpyDeviceToDevice, s));
-                    }
-                    HIPCHK(hipEventRecord(e0, s));
-                    run_pass(c, s, sd, false, 0.0, nullptr, true, false, false);
-                    HIPCHK(hipEventRecord(e1, s));
-                    HIPCHK(hipEventSynchronize(e1));
-                    float ms = 0.0f;
-                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                    if (r > 0) t = std::min(t, (double)ms);
-                }
-                if (t < best) {
-                    best = t;
-                    best_ratio = ratio;
-                    best_blocks = c->cells_ok ? per_cu : 0;
-                }
+            const double t = time_sweep();
+            if (t < best) {
+                best = t;
+                best_ratio = ratio;
+                best_blocks = c->cells_ok ? max_per_cu : 0;
             }
         }
-        c->prof = was_prof;
         c->cell_ratio = best_ratio;
         c->cells_per_cu = best_blocks;
         configure_grid(c);
+        // 2. the dense-owner kernel's waves per owner, where that kernel runs
+        // (owners whose neighbourhood outgrows the cell kernel's staging)
+        if (c->dense_on && c->cells_ok) {
+            const int waves[] = {4, 8, 16};
+            int best_waves = c->dense_waves;
+            double best_w = 1e300;
+            for (int w : waves) {
+                set_dense_waves(c, w);
+                const double t = time_sweep();
+                if (t < best_w) {
+                    best_w = t;
+                    best_waves = w;
+                }
+            }
+            set_dense_waves(c, best_waves);
+            best = std::min(best, best_w);
+        }
+        c->prof = was_prof;
         if (best_ms) *best_ms = best;
         if (cell_ratio) *cell_ratio = best_ratio;
         if (blocks_per_cu) *blocks_per_cu = best_blocks;
+    });
+}
+
+tb3_status tb3_get_tuning(const tb3_ctx *c, double *cell_ratio, int *blocks_per_cu, int *dense_waves)
+{
+    if (!c) return TB3_EINVAL;
+    if (cell_ratio) *cell_ratio = c->cell_ratio;
+    if (blocks_per_cu) *blocks_per_cu = c->cells_per_cu;
+    if (dense_waves) *dense_waves = c->dense_waves;
+    return TB3_OK;
+}
+
+tb3_status tb3_set_tuning(tb3_ctx *c, double cell_ratio, int dense_waves)
+{
+    if (!c) return TB3_EINVAL;
+    if (!(cell_ratio >= 1.0 && cell_ratio <= 4.0))
+        return fail(c, TB3_EINVAL, "cell_ratio must lie in [1, 4] (cells at least one cutoff wide)");
+    if (!p3bca_dense_waves_ok(dense_waves)) return fail(c, TB3_EINVAL, "dense_waves must be 4, 8 or 16");
+    return guarded(c, [&] {
+        set_dense_waves(c, dense_waves);
+        if (cell_ratio != c->cell_ratio) {
+            c->cell_ratio = cell_ratio;
+            if (c->n > 0 && uses_cutoff(c)) configure_grid(c);
+        }
     });
 }
 

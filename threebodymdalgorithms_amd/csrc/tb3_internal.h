@@ -165,11 +165,14 @@ int p3bca_cells_blocks_per_cu();
 // like the cell kernel's own overflow.  `owners` holds n ints.  Call
 // p3bca_dense_setup once per device before the first launch (it sets the
 // kernel's LDS limit, which a captured graph cannot).
+// `waves` (4, 8 or 16) is the workgroup size per owner in waves.
 void p3bca_dense_setup();
 int p3bca_dense_cap();
-int p3bca_dense_blocks_per_cu();
+bool p3bca_dense_waves_ok(int waves);
+int p3bca_dense_blocks_per_cu(int waves);
 void launch_p3bca_dense(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int grid);
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int grid,
+                        int waves);
 
 // All-triplet sweeps (NATA / AUTA, or P3BCA without a cutoff).
 void launch_auta_exact(hipStream_t s, const DeviceState &d, double nu, Status *st);

@@ -1508,7 +1508,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
 // ---------------------------------------------------------------------------
 // Dense-owner sweep: the owners the cell kernel could not stage.
 //
-// One owner per workgroup of DENSE_WAVES waves, taken from the queue the
+// One owner per workgroup of DW waves (4, 8 or 16: a tuning knob), taken from the queue the
 // cell kernel filled.  The owner's list is the same forward set the cell
 // kernel would have used (its own cell's later slots, then the forward
 // neighbour cells, within the cutoff), gathered from the cell-ordered
@@ -1517,8 +1517,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
 // the cell kernel.  A list longer than DCAP raises the overflow flags (the
 // host then re-runs the sweep with the per-owner kernel).
 
-constexpr int DENSE_WAVES = 8;
-constexpr int DCAP = 1440;  // with the 8 queues, two workgroups fit a CU's 160 KB of LDS
+constexpr int DCAP = 1440;  // two workgroups of 4 or 8 waves (one of 16) fit a CU's 160 KB of LDS
 
 struct DenseDev {
     CutDev c;
@@ -1530,8 +1529,8 @@ struct DenseDev {
 
 size_t dense_lds() { return (size_t)DCAP * (6 * sizeof(double) + sizeof(int)); }
 
-template <class T, bool GRAD>
-__global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
+template <class T, bool GRAD, int DW>
+__global__ __launch_bounds__(DW *WAVE) void k_p3bca_dense(DenseDev a)
 {
     extern __shared__ double dsm[];
     double *qx = dsm, *qy = qx + DCAP, *qz = qy + DCAP;
@@ -1540,11 +1539,11 @@ __global__ __launch_bounds__(DENSE_WAVES *WAVE) void k_p3bca_dense(DenseDev a)
     __shared__ int cst[32], cpre[33], sinfo[4];
     __shared__ int s_item, s_len;
     __shared__ double s_gi[3];
-    __shared__ unsigned dring[DENSE_WAVES][RQ];
+    __shared__ unsigned dring[DW][RQ];
 
     const CutDev &A = a.c;
     const int tid = threadIdx.x, lane = lane_id(), wv = tid / WAVE;
-    constexpr int NT = DENSE_WAVES * WAVE;
+    constexpr int NT = DW * WAVE;
     const T nu = (T)A.nu, K = (T)(0.375 * A.nu);
     const double c2 = A.c2;
     const int nowners = A.st->halt ? 0 : A.st->dense_n;  // written by the cell kernel before this launch
@@ -2617,28 +2616,55 @@ int p3bca_cells_blocks_per_cu()
     return nb;
 }
 
+template <int DW>
+void dense_setup_waves(size_t shm)
+{
+    allow_lds(k_p3bca_dense<double, true, DW>, shm);
+    allow_lds(k_p3bca_dense<double, false, DW>, shm);
+    allow_lds(k_p3bca_dense<float, true, DW>, shm);
+    allow_lds(k_p3bca_dense<float, false, DW>, shm);
+}
+
 void p3bca_dense_setup()
 {
     const size_t shm = dense_lds();
-    allow_lds(k_p3bca_dense<double, true>, shm);
-    allow_lds(k_p3bca_dense<double, false>, shm);
-    allow_lds(k_p3bca_dense<float, true>, shm);
-    allow_lds(k_p3bca_dense<float, false>, shm);
+    dense_setup_waves<4>(shm);
+    dense_setup_waves<8>(shm);
+    dense_setup_waves<16>(shm);
 }
 
 int p3bca_dense_cap() { return DCAP; }
 
-int p3bca_dense_blocks_per_cu()
+bool p3bca_dense_waves_ok(int waves) { return waves == 4 || waves == 8 || waves == 16; }
+
+int p3bca_dense_blocks_per_cu(int waves)
 {
+    const void *k = waves == 4    ? reinterpret_cast<const void *>(k_p3bca_dense<double, true, 4>)
+                    : waves == 16 ? reinterpret_cast<const void *>(k_p3bca_dense<double, true, 16>)
+                                  : reinterpret_cast<const void *>(k_p3bca_dense<double, true, 8>);
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(k_p3bca_dense<double, true>),
-                                                     DENSE_WAVES * WAVE, dense_lds()) != hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, waves * WAVE, dense_lds()) != hipSuccess || nb < 1)
         nb = 1;
     return nb;
 }
 
+template <int DW>
+void launch_dense_waves(hipStream_t s, const DenseDev &a, int mixed, int grad, unsigned grid)
+{
+    const dim3 gr(grid), blk(DW * WAVE);
+    const size_t shm = dense_lds();
+    if (mixed) {
+        if (grad) hipLaunchKernelGGL((k_p3bca_dense<float, true, DW>), gr, blk, shm, s, a);
+        else hipLaunchKernelGGL((k_p3bca_dense<float, false, DW>), gr, blk, shm, s, a);
+    } else {
+        if (grad) hipLaunchKernelGGL((k_p3bca_dense<double, true, DW>), gr, blk, shm, s, a);
+        else hipLaunchKernelGGL((k_p3bca_dense<double, false, DW>), gr, blk, shm, s, a);
+    }
+}
+
 void launch_p3bca_dense(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int grid)
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int grid,
+                        int waves)
 {
     if (d.n <= 0) return;
     DenseDev a;
@@ -2650,15 +2676,10 @@ void launch_p3bca_dense(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     a.cell_of = g.cell_of;
     a.owners = owners;
     a.queue = &st->dqueue[0];
-    const dim3 gr((unsigned)(grid < 1 ? 1 : grid)), blk(DENSE_WAVES * WAVE);
-    const size_t shm = dense_lds();
-    if (mixed) {
-        if (grad) hipLaunchKernelGGL((k_p3bca_dense<float, true>), gr, blk, shm, s, a);
-        else hipLaunchKernelGGL((k_p3bca_dense<float, false>), gr, blk, shm, s, a);
-    } else {
-        if (grad) hipLaunchKernelGGL((k_p3bca_dense<double, true>), gr, blk, shm, s, a);
-        else hipLaunchKernelGGL((k_p3bca_dense<double, false>), gr, blk, shm, s, a);
-    }
+    const unsigned gr = (unsigned)(grid < 1 ? 1 : grid);
+    if (waves == 4) launch_dense_waves<4>(s, a, mixed, grad, gr);
+    else if (waves == 16) launch_dense_waves<16>(s, a, mixed, grad, gr);
+    else launch_dense_waves<8>(s, a, mixed, grad, gr);
 }
 
 void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,

@@ -211,12 +211,25 @@ class Engine:
         return int(nt.value) if want_count else None
 
     def autotune(self, reps=3):
-        """Pick the cell width / cutoff ratio and the cell kernel's resident
-        workgroups per CU by timing force sweeps of the current particles."""
+        """Pick the cell width / cutoff ratio, and the dense-owner kernel's
+        waves per owner where that kernel runs, by timing force sweeps of the
+        current particles; returns the best sweep time and every knob."""
         ms, ratio, blocks = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int(0)
         self._check(self._lib.tb3_autotune(self._ctx, int(reps), ctypes.byref(ms), ctypes.byref(ratio),
                                            ctypes.byref(blocks)))
-        return {"sweep_ms": ms.value, "cell_ratio": ratio.value, "blocks_per_cu": blocks.value}
+        return {"sweep_ms": ms.value, **self.get_tuning()}
+
+    def get_tuning(self):
+        """The knobs in force: cell width / cutoff ratio, the cell kernel's
+        resident workgroups per CU, the dense-owner kernel's waves per owner."""
+        ratio, blocks, waves = ctypes.c_double(0.0), ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self._lib.tb3_get_tuning(self._ctx, ctypes.byref(ratio), ctypes.byref(blocks),
+                                             ctypes.byref(waves)))
+        return {"cell_ratio": ratio.value, "blocks_per_cu": blocks.value, "dense_waves": waves.value}
+
+    def set_tuning(self, cell_ratio=1.0, dense_waves=8):
+        """Fix the knobs (cell_ratio in [1, 4]; dense_waves 4, 8 or 16)."""
+        self._check(self._lib.tb3_set_tuning(self._ctx, float(cell_ratio), int(dense_waves)))
 
     # -- phase timing ---------------------------------------------------------
     def set_profiling(self, enable=True):
