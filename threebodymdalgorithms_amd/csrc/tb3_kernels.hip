@@ -1907,9 +1907,11 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
         m = split;
         m2 = 0;
     }
-    // len <= 2048 (the list and its positions fill at most 64 KB of LDS), so
-    // item numbers, row starts and (2m - 1)^2 stay below 2^24: exact in int
-    // and in FP32
+    // Item numbers and row starts fit an int (lists hold at most 5851
+    // entries: 160 KB of LDS).  Up to 2048 entries (64 KB) they and
+    // (2m - 1)^2 stay below 2^24, exact in FP32, so the row estimates below
+    // are off by less than 0.001; on longer lists the correction loops may
+    // run more than once.
     const int nitems = ROLE == 1 ? m * m2 : m * (m - 1) / 2;
     acc = 0.0;  // lanes 0, 1, 2 hold x, y, z
     int head = 0, qn = 0;
@@ -1950,9 +1952,8 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
         if (ok) {
             // the row from an FP32 estimate (within one of the true row),
             // then corrected with exact integer comparisons
-            // (the estimates are off by less than 0.001, so the loops below
-            // run at most once; they are kept as loops, not unrolled, so the
-            // decode stays exact whatever the estimate)
+            // (the loops below correct the estimate exactly, whatever its
+            // error; they are kept as loops, not unrolled: they rarely run)
             if constexpr (ROLE == 1) {
                 int ra = (int)((float)t * inv_m2);
                 int rem = t - ra * m2;
@@ -1998,7 +1999,7 @@ __device__ void exact_role_staged(const StagedList &L, double xp, double yp, dou
         const unsigned long long mk = __ballot(ok);
         if (ok) {
             const int q = (head + qn + lane_prefix(mk)) & (RING - 1);
-            sh.qa[q] = ia | (ib << 16);  // list indices are below 2048
+            sh.qa[q] = ia | (ib << 16);  // list indices are below 2^16
         }
         const int got = __popcll(mk);
         qn += got;
