@@ -133,28 +133,34 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count(), "cpus_available": avail}
 
 
-def force_errors(f, fo):
-    """Per-particle force error of f against fo: the pure ratio |df|/|fo|
-    (max and RMS) and the ratio with |fo| floored at the RMS force."""
+def force_errors(f, fo, scale):
+    """Per-particle force error of f against fo: the ratio to each particle's
+    force scale S_p (the summed lengths of its triplet contributions, which
+    bounds the error of a sum of terms each computed to a relative error e by
+    e S_p however much they cancel), the pure ratio |df|/|fo| (max and RMS)
+    and the ratio with |fo| floored at the RMS force."""
     num = np.linalg.norm(f - fo, axis=1)
     den = np.linalg.norm(fo, axis=1)
     rel = num / np.where(den > 0, den, 1.0)
     floor = np.sqrt(np.mean(den ** 2))
-    return {"max_rel_force_err": float(rel.max()), "rms_rel_force_err": float(np.sqrt(np.mean(rel ** 2))),
+    return {"max_rel_force_err_scale": float((num / np.where(scale > 0, scale, 1.0)).max()),
+            "max_rel_force_err": float(rel.max()), "rms_rel_force_err": float(np.sqrt(np.mean(rel ** 2))),
             "max_rel_force_err_floored": float((num / np.maximum(den, floor)).max())}
 
 
 def accuracy_vs_exact(eng, P):
     """FP64 and FP32-mixed forces and energies of the initial state against
-    the bit-exact FP64 path (whose forces equal the reference's bit for bit)."""
+    the bit-exact FP64 path (whose forces equal the reference's bit for bit),
+    with each particle's force scale from tb3_force_scale."""
     eng.set_particles(P)
     eng.configure("p3bca", cutoff=CUTOFF, precision="exact")
     Ex, fx, _, _ = eng.energy_grad()
+    scale = eng.force_scale()
     out = {}
     for prec in ("fp64", "fp32"):
         eng.configure("p3bca", cutoff=CUTOFF, precision=prec)
         E, f, _, _ = eng.energy_grad()
-        out[prec] = {**force_errors(f, fx), "rel_energy_err": float(abs(E - Ex) / abs(Ex))}
+        out[prec] = {**force_errors(f, fx, scale), "rel_energy_err": float(abs(E - Ex) / abs(Ex))}
     return out
 
 

@@ -200,6 +200,15 @@ tb3_status tb3_energy_grad_dev_noalloc(tb3_ctx *ctx, int64_t n, const double *d_
  * knob.  The all-triplet sweeps have nothing to tune and return at once. */
 tb3_status tb3_autotune(tb3_ctx *ctx, int reps, double *best_ms, double *cell_ratio, int *blocks_per_cu);
 
+/* Each particle's force scale S_p: the sum over p's triplets of the length of
+ * p's contribution, |f_p| <= S_p (the condition of the force sum; FP64, from
+ * a sweep of a copy of the held positions).  A sum of terms computed with
+ * relative error e each is off by at most e S_p however much the terms
+ * cancel, so S_p is the yardstick of the FP32-mixed path's force error.
+ * scale holds n doubles.  P3BCA with a positive cutoff on one system;
+ * TB3_EUNSUPPORTED otherwise.  No counterpart in the reference. */
+tb3_status tb3_force_scale(tb3_ctx *ctx, double *scale);
+
 /* The tuned knobs (each output may be NULL): cell width / cutoff ratio, the
  * cell kernel's resident workgroups per CU, and the dense-owner kernel's
  * waves per owner.  No counterpart in the reference. */

@@ -57,6 +57,7 @@ def test_null_context_is_einval():
     assert lib.tb3_device_count(None) == _abi.EINVAL
     assert lib.tb3_create(0, None) == _abi.EINVAL
     assert lib.tb3_get_tuning(None, None, None, None) == _abi.EINVAL
+    assert lib.tb3_force_scale(None, None) == _abi.EINVAL
     assert lib.tb3_set_tuning(None, 1.0, 8) == _abi.EINVAL
 
 

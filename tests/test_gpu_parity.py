@@ -295,6 +295,33 @@ def test_fp32_mixed_one_sweep(eng, oracle, algo, n, box, cutoff):
     assert e["rms_rel"] <= 1e-5
 
 
+@pytest.mark.parametrize("n,box,cutoff,moved", [(4000, 12.6, 2.0, False), (3000, 12.0, 2.0, True),
+                                                (300, 4.0, 2.2, False)])
+def test_force_scale_matches_oracle(eng, oracle, n, box, cutoff, moved):
+    """tb3_force_scale (the squared-distance form, FP64) against the oracle's
+    sum of the reference's contribution lengths; both bound |f| from above.
+    Also particles outside the box and a grid with fewer than 3 cells."""
+    P = uniform_particles(n, box, 900 + n)
+    L, _ = oracle.domain_size(P)
+    if moved:
+        P = P.copy()
+        P[::7, 0] += L[0]
+    eng.set_particles(P)
+    eng.configure("p3bca", cutoff=cutoff, domain_L=L, precision="fp64")
+    say(f"force scale n={n} box={box} moved={moved}")
+    s = eng.force_scale()
+    so = oracle.force_scale(P, cutoff=cutoff, L=L)
+    *_, fo, _ = oracle.sweep(P, cutoff=cutoff, L=L, mode="owner")
+    rel = np.abs(s - so) / np.where(so > 0, so, 1.0)
+    say(f"  max relative difference {rel.max():.2e}")
+    assert rel.max() <= 1e-9
+    assert np.all(s >= np.linalg.norm(fo, axis=1) * (1 - 1e-9))
+    assert np.array_equal(s == 0, so == 0)
+    eng.configure("auta")
+    with pytest.raises(tb3.TB3Error):
+        eng.force_scale()
+
+
 @pytest.mark.parametrize("precision", ["exact", "fp64", "fp32"])
 def test_energy_and_adjoint(eng, oracle, precision):
     n = 1500

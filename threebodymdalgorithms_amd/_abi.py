@@ -64,6 +64,7 @@ SIGNATURES = {
     "tb3_energy_grad_dev_noalloc": (ctypes.c_int, [_ctxp, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double,
                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _u64p]),
     "tb3_autotune": (ctypes.c_int, [_ctxp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
+    "tb3_force_scale": (ctypes.c_int, [_ctxp, _dp]),
     "tb3_get_tuning": (ctypes.c_int, [_ctxp, _dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "tb3_set_tuning": (ctypes.c_int, [_ctxp, ctypes.c_double, ctypes.c_int]),
     "tb3_set_profiling": (ctypes.c_int, [_ctxp, ctypes.c_int]),

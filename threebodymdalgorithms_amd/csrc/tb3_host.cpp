@@ -1128,6 +1128,46 @@ static double energy_on_copy(tb3_ctx *c, unsigned long long *count)
     return c->h_status->energy;
 }
 
+tb3_status tb3_force_scale(tb3_ctx *c, double *scale)
+{
+    if (!c) return TB3_EINVAL;
+    if (!scale && c->n > 0) return fail(c, TB3_EINVAL, "scale is required");
+    tb3_status st = check_ready(c);
+    if (st != TB3_OK) return st;
+    if (!uses_cutoff(c) || c->nrep > 1)
+        return fail(c, TB3_EUNSUPPORTED, "the force scale serves one P3BCA system with a positive cutoff");
+    return guarded(c, [&] {
+        if (c->n == 0) return;
+        hipStream_t s = c->stream;
+        c->scratch.ensure(c->n);
+        DeviceState sd = c->scratch.d;
+        const DeviceState &d = c->state.d;
+        const size_t b = (size_t)c->n * sizeof(double);
+        HIPCHK(hipMemcpyAsync(sd.x, d.x, b, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(sd.y, d.y, b, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(sd.z, d.z, b, hipMemcpyDeviceToDevice, s));
+        // the per-owner kernel with the list capacity grown until no list overflows
+        for (int attempt = 0;; ++attempt) {
+            launch_prepare(c, s, sd, false, 0.0, false);
+            launch_zero(s, sd.fx, sd.n);
+            launch_zero(s, sd.fy, sd.n);
+            launch_zero(s, sd.fz, sd.n);
+            if (sd.n >= 3)
+                launch_p3bca_fast(s, sd, c->grid, cutoff_spec(c), c->nu, c->list_cap, 0, 2, c->d_status, 0, 0);
+            launch_finalize(s, c->d_status);
+            read_status(c, s);
+            if (!c->h_status->flags[2]) break;
+            const int need = c->h_status->flags[1];
+            if (attempt >= 8 || need <= c->list_cap || need > MAX_LIST ||
+                p3bca_fast_lds(next_pow2(need + need / 8)) > 160 * 1024)
+                throw Fail(TB3_EUNSUPPORTED, "a particle has too many neighbours within the cutoff for the force scale");
+            c->list_cap = std::min(next_pow2(need + need / 8), MAX_LIST);
+        }
+        HIPCHK(hipMemcpyAsync(scale, sd.fx, b, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
 tb3_status tb3_energy(tb3_ctx *c, double *E, uint64_t *n_triplets)
 {
     if (!c || !E) return fail(c, TB3_EINVAL, "E is required");

@@ -145,6 +145,8 @@ void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const do
 //          atomically into fx/fy/fz (zeroed by launch_cells_build), energy
 //          and count into st.  mixed = FP32 per-triplet arithmetic.
 //   exact: bit-exact ordered role sums into facc; count into st.
+// launch_p3bca_fast with grad = 2 (FP64) writes each particle's force scale
+// (the summed lengths of its triplet contributions) into fx instead.
 void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
                        int cap, int mixed, int grad, Status *st, int count_slot, int want_energy);
 void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,

@@ -844,7 +844,10 @@ __device__ __forceinline__ int owner_slot(int b, int n)
 // accumulators (LDS atomics), flushed to global memory with one atomic per
 // neighbour and component at the end.
 
-template <class T, bool GRAD>
+// SCALE (with T = double and GRAD): instead of the gradient, each particle's
+// force scale, the sum over its triplets of the length of its contribution
+// (the condition of its force sum), into fx (fy and fz stay zero).
+template <class T, bool GRAD, bool SCALE = false>
 __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
 {
     extern __shared__ double smem[];
@@ -929,7 +932,18 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
             double cu, cv, cw;
             T et;
             at_coeffs_from64<T, GRAD>(u, v, w, nu, K, cu, cv, cw, et);
-            if (GRAD) {
+            if (GRAD && SCALE) {
+                // lengths of the three gradients (each particle's contribution)
+                const double tx = cw * ex, ty = cw * ey, tz = cw * ez;
+                const double jx = __builtin_fma(cu, djx, -tx), jy = __builtin_fma(cu, djy, -ty),
+                             jz = __builtin_fma(cu, djz, -tz);
+                const double kx = __builtin_fma(cv, dkx, tx), ky = __builtin_fma(cv, dky, ty),
+                             kz = __builtin_fma(cv, dkz, tz);
+                const double ix = jx + kx, iy = jy + ky, iz = jz + kz;
+                gpx += __builtin_sqrt(ix * ix + iy * iy + iz * iz);
+                lds_add(&gx_[ia], __builtin_sqrt(jx * jx + jy * jy + jz * jz));
+                lds_add(&gx_[ib], __builtin_sqrt(kx * kx + ky * ky + kz * kz));
+            } else if (GRAD) {
                 gpx = __builtin_fma(-cu, djx, __builtin_fma(-cv, dkx, gpx));
                 gpy = __builtin_fma(-cu, djy, __builtin_fma(-cv, dky, gpy));
                 gpz = __builtin_fma(-cu, djz, __builtin_fma(-cv, dkz, gpz));
@@ -2560,6 +2574,9 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
             allow_lds(k_p3bca_fast<float, false>, shm);
             hipLaunchKernelGGL((k_p3bca_fast<float, false>), grid, blk, shm, s, a);
         }
+    } else if (grad == 2) {
+        allow_lds(k_p3bca_fast<double, true, true>, shm);
+        hipLaunchKernelGGL((k_p3bca_fast<double, true, true>), grid, blk, shm, s, a);
     } else {
         if (grad) {
             allow_lds(k_p3bca_fast<double, true>, shm);

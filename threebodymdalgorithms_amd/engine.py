@@ -219,6 +219,13 @@ class Engine:
                                            ctypes.byref(blocks)))
         return {"sweep_ms": ms.value, **self.get_tuning()}
 
+    def force_scale(self):
+        """Each particle's force scale: the summed lengths of its triplet
+        contributions (P3BCA with a cutoff), the yardstick of force errors."""
+        out = np.zeros(self.n)
+        self._check(self._lib.tb3_force_scale(self._ctx, _dp(out)))
+        return out
+
     def get_tuning(self):
         """The knobs in force: cell width / cutoff ratio, the cell kernel's
         resident workgroups per CU, the dense-owner kernel's waves per owner."""
