@@ -383,7 +383,7 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
         dev_alloc(g.cell_cnt, g.ncell);
         HIPCHK(hipMemset(g.cell_cnt, 0, (size_t)g.ncell * sizeof(int)));
         dev_alloc(g.cell_start, (long long)g.ncell + 1);
-        dev_alloc(g.order, g.ncell);
+        dev_alloc(g.order, cell_order_len(g.ncell));
         dev_alloc(g.cell_work, g.ncell);
         HIPCHK(hipMemset(g.cell_work, 0, (size_t)g.ncell * sizeof(unsigned)));
         dev_alloc(g.scatter_ticket, 1);
@@ -490,7 +490,8 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
 void launch_tail(tb3_ctx *c, hipStream_t s, DeviceState &d, double dt, const double *g, bool grad, bool record)
 {
     const bool sum_roles = c->precision == TB3_FP64_EXACT && grad && d.n >= 3;
-    launch_update_fin(s, d, dt * 0.5, g, c->d_status, sum_roles ? 1 : 0, record ? 1 : 0);
+    const bool cells = uses_cutoff(c) && d.n >= 3 && c->grid.order;
+    launch_update_fin(s, d, dt * 0.5, g, c->d_status, sum_roles ? 1 : 0, record ? 1 : 0, cells ? &c->grid : nullptr);
     HIPCHK(hipGetLastError());
 }
 
@@ -1352,7 +1353,7 @@ size_t noalloc_layout(unsigned char *base, long long n, int ncell, NoallocLayout
     l.g.zs = reinterpret_cast<double *>(take(dn));
     l.g.cell_cnt = reinterpret_cast<int *>(take(cn));
     l.g.cell_start = reinterpret_cast<int *>(take(cn + sizeof(int)));
-    l.g.order = reinterpret_cast<int *>(take(cn));
+    l.g.order = reinterpret_cast<int *>(take((size_t)cell_order_len(std::max(ncell, 1)) * sizeof(int)));
     l.g.cell_work = reinterpret_cast<unsigned *>(take(cn));
     l.g.scatter_ticket = reinterpret_cast<unsigned *>(take(sizeof(unsigned)));
     l.scan_tmp = reinterpret_cast<long long *>(

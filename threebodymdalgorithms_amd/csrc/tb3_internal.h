@@ -21,6 +21,18 @@ struct DeviceState {
 // Uniform cell grid over the periodic domain.  Cell width >= cutoff, so the
 // 27 cells around a particle's cell (fewer when a dimension has < 3 cells)
 // hold every particle within the periodic cutoff.
+// The cell kernel's dispatch order: one contiguous run of cells per XCD, the
+// runs cut to about equal work (so their lengths vary around ncell / 8),
+// each sorted heaviest first.  `order` holds 8 runs of cell_run_cap(ncell)
+// entries, then the 8 run lengths, then the 9 cuts (run r is cells
+// [cut r, cut r+1)) that the last replayed step's update computed.
+__host__ __device__ inline int cell_run_cap(int ncell)
+{
+    const int run = (ncell + 7) / 8;
+    return run + (run + 1) / 2;
+}
+__host__ __device__ inline long long cell_order_len(int ncell) { return 8LL * cell_run_cap(ncell) + 17; }
+
 struct CellGrid {
     int nc[3] = {1, 1, 1};
     int ncell = 1;
@@ -109,8 +121,10 @@ void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h);
 // update skipped entirely when st->flags[2] (overflow) is set.  With record
 // (inside a replayed step graph) a step that did not halt also appends its
 // count to st->counts and clears the status block for the next replay.
+// With `cells` (a cell list this pass swept), one more workgroup cuts the
+// next sweep's XCD runs from the per-cell work.
 void launch_update_fin(hipStream_t s, const DeviceState &d, double hdt, const double g[3], Status *st, int sum_roles,
-                       int record);
+                       int record, const CellGrid *cells);
 void launch_sum_roles(hipStream_t s, const DeviceState &d);
 void launch_zero(hipStream_t s, double *p, long long count);
 

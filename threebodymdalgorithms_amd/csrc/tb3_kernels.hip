@@ -232,6 +232,129 @@ __global__ void k_predict(DeviceState d, double dt, double h)
     predict_one(d, p, dt, h);
 }
 
+// The dispatch order of the cell kernel (see cell_run_cap), for it to start
+// the longest cells first: with under two workgroups per slot, the cells that
+// start last otherwise decide when the sweep ends.  Each cell's work is its
+// triplet count in the previous sweep (cell_work: most particles move little
+// in a step), or 16 n^2 for a cell that has none yet (n particles; start(c)
+// is cell c's first slot).
+//
+// The cuts: cell c joins run floor(8 (W_c + w_c / 2) / W), W_c being the
+// work of the cells before it and W the total, which gives contiguous runs
+// of about W / 8 each, so that the XCDs finish together (runs of equal cell
+// counts left one XCD with 12% more work than the mean on a uniform
+// system).  Should a run exceed the capacity, every run takes ncell / 8 cells
+// instead.  One workgroup of 256 threads; the 9 cuts go to out.  LDS: fpart
+// (4 floats), cut (9 ints).
+template <class Start>
+__device__ void compute_cuts(int ncell, const Start &start, const unsigned *__restrict__ cell_work, int *out,
+                             float *fpart, int *cut)
+{
+    const int t = threadIdx.x;
+    const int run = (ncell + 7) >> 3, cap = cell_run_cap(ncell);
+    auto work = [&](int c) {
+        const unsigned w = cell_work[c];
+        if (w) return (float)w;
+        const float n = (float)(start(c + 1) - start(c));
+        return 16.0f * n * n;
+    };
+    // each thread sums a chunk of cells (their work kept in registers, up to
+    // WCACHE of them), a scan of the chunk sums (in-wave shuffles, then the
+    // four wave totals), then each cell's run
+    constexpr int WCACHE = 32;  // chunks of up to 8192 cells stay in registers
+    const int per = (ncell + 255) >> 8;
+    const int c0 = min(t * per, ncell), c1 = min(c0 + per, ncell);
+    float wc[WCACHE];
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < WCACHE; ++k) {
+        wc[k] = c0 + k < c1 ? work(c0 + k) : 0.0f;
+        s += wc[k];
+    }
+    for (int c = c0 + WCACHE; c < c1; ++c) s += work(c);
+    float incl = s;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const float u = __shfl_up(incl, o);
+        if ((t & (WAVE - 1)) >= o) incl += u;
+    }
+    if ((t & (WAVE - 1)) == WAVE - 1) fpart[t >> 6] = incl;
+    if (t <= 8) cut[t] = ncell;
+    __syncthreads();
+    float before = 0.0f;
+    for (int w = 0; w < (t >> 6); ++w) before += fpart[w];
+    const float total = (fpart[0] + fpart[1]) + (fpart[2] + fpart[3]);
+    if (total > 0.0f) {
+        float acc = before + incl - s;
+        for (int k = 0; c0 + k < c1; ++k) {
+            const int c = c0 + k;
+            const float w = k < WCACHE ? wc[k] : work(c);
+            const int r = min(7, max(0, (int)(8.0f * (acc + 0.5f * w) / total)));
+            atomicMin(&cut[r], c);  // the first cell of run r (runs are contiguous)
+            acc += w;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        // run r starts at its first cell, or where the next non-empty run does
+        bool ok = total > 0.0f;
+        cut[0] = 0;
+        for (int r = 7; r >= 1; --r) cut[r] = min(cut[r], cut[r + 1]);
+        for (int r = 0; r < 8 && ok; ++r) ok = cut[r + 1] - cut[r] <= cap;
+        for (int r = 0; r <= 8; ++r) out[r] = ok ? cut[r] : min(r * run, ncell);
+    }
+}
+
+// The order itself: runs r0, r0 + rstep, ... < 8 of the workgroup, each
+// sorted heaviest first by a counting sort on bins of 1/16 octave of the
+// work.  The cuts are those the last integrating pass's update computed
+// (order[8 cap + 8 ...]), which the caller loaded into cut[0..8] early
+// (overlapping that load with its own), if they fit this grid; else each
+// run takes ncell / 8 cells.  256 threads; LDS: hist (256 ints), cut (9
+// ints).
+template <class Start>
+__device__ void order_runs(int ncell, const Start &start, const unsigned *__restrict__ cell_work,
+                           int *__restrict__ order, int *hist, int *cut, int r0, int rstep)
+{
+    if (r0 >= 8) return;
+    const int t = threadIdx.x;
+    const int run = (ncell + 7) >> 3, cap = cell_run_cap(ncell);
+    auto bin = [&](int c) {
+        const unsigned w = cell_work[c];
+        const float n = (float)(start(c + 1) - start(c));
+        const float k = w ? (float)w : 16.0f * n * n;
+        return 255 - min(255, (int)(16.0f * __log2f(1.0f + k)));
+    };
+    __syncthreads();  // the caller's cut[] stores
+    if (t == 0) {
+        bool ok = cut[0] == 0 && cut[8] == ncell;
+        for (int r = 0; r < 8 && ok; ++r) ok = cut[r + 1] >= cut[r] && cut[r + 1] - cut[r] <= cap;
+        if (!ok)
+            for (int r = 0; r <= 8; ++r) cut[r] = min(r * run, ncell);
+    }
+    __syncthreads();
+    for (int r = r0; r < 8; r += rstep) {
+        const int lo = cut[r], hi = cut[r + 1];
+        hist[t] = 0;
+        __syncthreads();
+        for (int c = lo + t; c < hi; c += 256) atomicAdd(&hist[bin(c)], 1);
+        __syncthreads();
+        // exclusive scan of the 256 bins, one per thread (Hillis-Steele)
+        const int own = hist[t];
+        for (int o = 1; o < 256; o <<= 1) {
+            const int u = t >= o ? hist[t - o] : 0;
+            __syncthreads();
+            hist[t] += u;
+            __syncthreads();
+        }
+        hist[t] -= own;
+        __syncthreads();
+        for (int c = lo + t; c < hi; c += 256) order[r * cap + atomicAdd(&hist[bin(c)], 1)] = c;
+        if (t == 0) order[8 * cap + r] = hi - lo;
+        __syncthreads();
+    }
+}
+
 // The last kernel of an integrating pass.  Workgroup 0 first folds the
 // sweep's slot sums into the status block (k_finalize); with `record` (a
 // replayed step graph) it then appends the step's count and, unless the
@@ -243,8 +366,19 @@ __global__ void k_predict(DeviceState d, double dt, double h)
 // status clear cannot affect that test, since it happens only when
 // flags[2] is already zero.
 __global__ __launch_bounds__(NSLOT) void k_update_fin(DeviceState d, double hdt, double gx, double gy, double gz,
-                                                     Status *st, int sum_roles, int record)
+                                                     Status *st, int sum_roles, int record, int ncell,
+                                                     const int *__restrict__ cell_start,
+                                                     const unsigned *__restrict__ cell_work, int *__restrict__ order)
 {
+    // with a cell list (ncell > 0) the last workgroup cuts the next sweep's
+    // XCD runs from this sweep's per-cell work
+    if (ncell > 0 && blockIdx.x == gridDim.x - 1) {
+        __shared__ float fpart[4];
+        __shared__ int cut[9];
+        compute_cuts(ncell, [&](int c) { return cell_start[c]; }, cell_work, order + 8 * cell_run_cap(ncell) + 8,
+                     fpart, cut);
+        return;
+    }
     if (blockIdx.x == 0) {
         fold_slots(st);
         __syncthreads();  // thread 0's fold (and halt) is visible to the workgroup
@@ -494,15 +628,6 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
     else atomicAdd(&cell_cnt[c], 1);
 }
 
-// Orders the cells of each XCD's run of (ncell + 7) / 8 consecutive cells
-// heaviest first into `order` (run r at r * run + rank), for the cell kernel
-// to start the longest cells first: with under two workgroups per slot, the
-// cells that start last otherwise decide when the sweep ends.  A cell's work
-// is the triplet count of its last sweep (most particles move little in a
-// step), or 16 n^2 for n particles before it has one.  A counting sort on
-// 1/16-octave bins of the work (256 bins; ties in any order), one workgroup
-// per run: the calling workgroup sorts runs r0, r0 + rstep, ... below 8.
-// start(c) is cell c's first slot.
 // The end of one replayed step fused with the start of the next (cutoff
 // algorithm, grids of at most FUSED_SCAN_CELLS cells): k_update_fin's fold,
 // count record, status clear and Particle::Update (structs.hpp:57-99), then,
@@ -516,8 +641,19 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
 __global__ __launch_bounds__(NSLOT) void k_update_bin(DeviceState d, double hdt, double gx, double gy, double gz,
                                                      Status *st, int sum_roles, double dt, double h, int zero_f,
                                                      GridDev g, int *__restrict__ cell_of,
-                                                     int *__restrict__ cell_cnt, int *__restrict__ cell_rank)
+                                                     int *__restrict__ cell_cnt, int *__restrict__ cell_rank,
+                                                     int ncell, const int *__restrict__ cell_start,
+                                                     const unsigned *__restrict__ cell_work, int *__restrict__ order)
 {
+    // the last workgroup cuts the next sweep's XCD runs from this sweep's
+    // per-cell work (the cell starts are the ones this sweep used)
+    if (blockIdx.x == gridDim.x - 1) {
+        __shared__ float fpart[4];
+        __shared__ int cut[9];
+        compute_cuts(ncell, [&](int c) { return cell_start[c]; }, cell_work, order + 8 * cell_run_cap(ncell) + 8,
+                     fpart, cut);
+        return;
+    }
     if (blockIdx.x == 0) {
         fold_slots(st);
         __syncthreads();  // thread 0's fold (and halt) is visible to the workgroup
@@ -580,39 +716,6 @@ __global__ __launch_bounds__(NSLOT) void k_update_bin(DeviceState d, double hdt,
     cell_rank[p] = atomicAdd(&cell_cnt[c], 1);
 }
 
-template <class Start>
-__device__ void order_runs(int ncell, const Start &start, const unsigned *__restrict__ cell_work,
-                           int *__restrict__ order, int *hist, int r0, int rstep)
-{
-    const int t = threadIdx.x;
-    const int run = (ncell + 7) >> 3;
-    auto bin = [&](int c) {
-        const int n = start(c + 1) - start(c);
-        const unsigned w = cell_work[c];
-        const float k = w ? (float)w : 16.0f * (float)n * (float)n;
-        return 255 - min(255, (int)(16.0f * __log2f(1.0f + k)));
-    };
-    for (int r = r0; r < 8; r += rstep) {
-        const int lo = r * run, hi = min(lo + run, ncell);
-        hist[t] = 0;
-        __syncthreads();
-        for (int c = lo + t; c < hi; c += 256) atomicAdd(&hist[bin(c)], 1);
-        __syncthreads();
-        // exclusive scan of the 256 bins, one per thread (Hillis-Steele)
-        const int own = hist[t];
-        for (int o = 1; o < 256; o <<= 1) {
-            const int u = t >= o ? hist[t - o] : 0;
-            __syncthreads();
-            hist[t] += u;
-            __syncthreads();
-        }
-        hist[t] -= own;
-        __syncthreads();
-        for (int c = lo + t; c < hi; c += 256) order[lo + atomicAdd(&hist[bin(c)], 1)] = c;
-        __syncthreads();
-    }
-}
-
 // Places each particle in its cell's range (index and a cell-ordered copy of
 // its position), counting the cell's counter back down to zero so the
 // counters are clear for the next step; also orders the cells (order_runs).
@@ -624,8 +727,9 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
                                                       double *__restrict__ ys, double *__restrict__ zs, int ncell,
                                                       int *__restrict__ order, const unsigned *__restrict__ cell_work)
 {
-    __shared__ int hist[256];
-    order_runs(ncell, [&](int c) { return cell_start[c]; }, cell_work, order, hist, blockIdx.x, gridDim.x);
+    __shared__ int hist[256], cut[9];
+    if (threadIdx.x <= 8) cut[threadIdx.x] = order[8 * cell_run_cap(ncell) + 8 + threadIdx.x];
+    order_runs(ncell, [&](int c) { return cell_start[c]; }, cell_work, order, hist, cut, blockIdx.x, gridDim.x);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int c = cell_of[p];
@@ -658,7 +762,7 @@ __global__ __launch_bounds__(256) void k_cell_scatter_fused(
 {
     __shared__ int lst[FUSED_SCAN_CELLS + 1];
     __shared__ int part[256];
-    __shared__ int hist[256];
+    __shared__ int hist[256], cut[9];
     __shared__ int s_last;
     // A halted step-graph replay keeps the cell list of the step that
     // overflowed (the host finishes that step on it); every workgroup reads
@@ -666,8 +770,9 @@ __global__ __launch_bounds__(256) void k_cell_scatter_fused(
     if (st->halt) return;
     const int t = threadIdx.x;
     const int p = blockIdx.x * blockDim.x + t;  // >= n in the ordering workgroups
-    int pcell = 0, prank = 0;
+    int pcell = 0, prank = 0, gcut = 0;
     double px = 0.0, py = 0.0, pz = 0.0;
+    if (blockIdx.x >= nsc && t <= 8) gcut = order[8 * cell_run_cap(ncell) + 8 + t];  // the given cuts
     if (p < n) {
         pcell = cell_of[p];
         prank = cell_rank[p];
@@ -702,7 +807,10 @@ __global__ __launch_bounds__(256) void k_cell_scatter_fused(
     __syncthreads();
     if (blockIdx.x == 0)
         for (int c = t; c <= ncell; c += 256) cell_start[c] = lst[c];
-    if (blockIdx.x >= nsc) order_runs(ncell, [&](int c) { return lst[c]; }, cell_work, order, hist, blockIdx.x - nsc, 8);
+    if (blockIdx.x >= nsc) {
+        if (t <= 8) cut[t] = gcut;
+        order_runs(ncell, [&](int c) { return lst[c]; }, cell_work, order, hist, cut, blockIdx.x - nsc, 8);
+    }
 
     if (p < n) {
         const int slot = lst[pcell] + prank;
@@ -1166,13 +1274,14 @@ __device__ __forceinline__ int staged_slot(const int *cst, const int *cpre, int 
 
 // Cell for workgroup b, or -1: workgroups are dispatched round-robin over
 // the 8 XCDs (b % 8), so each XCD gets a contiguous run of cells, whose
-// neighbourhoods overlap in its own L2, and takes them heaviest first.
+// neighbourhoods overlap in its own L2, of about an eighth of the work, and
+// takes them heaviest first (order_runs).
 __device__ __forceinline__ int cell_for_block(int b, int ncell, const int *order)
 {
-    const int run = (ncell + 7) >> 3;
+    const int cap = cell_run_cap(ncell);
     const int r = b & 7, k = b >> 3;
-    if (k >= run || r * run + k >= ncell) return -1;
-    return order[r * run + k];
+    if (k >= cap || k >= order[8 * cap + r]) return -1;
+    return order[r * cap + k];
 }
 
 template <class T, bool GRAD, bool WANT_E>
@@ -2466,10 +2575,13 @@ void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h)
 }
 
 void launch_update_fin(hipStream_t s, const DeviceState &d, double hdt, const double g[3], Status *st, int sum_roles,
-                       int record)
+                       int record, const CellGrid *cells)
 {
-    const unsigned nb = d.n > 0 ? blocks_for(d.n, NSLOT) : 1;
-    hipLaunchKernelGGL(k_update_fin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g[0], g[1], g[2], st, sum_roles, record);
+    // the particles' workgroups, then (with a cell list) one that cuts the next sweep's XCD runs
+    const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + (cells ? 1 : 0);
+    hipLaunchKernelGGL(k_update_fin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g[0], g[1], g[2], st, sum_roles, record,
+                       cells ? cells->ncell : 0, cells ? cells->cell_start : nullptr,
+                       cells ? cells->cell_work : nullptr, cells ? cells->order : nullptr);
 }
 
 void launch_sum_roles(hipStream_t s, const DeviceState &d)
@@ -2523,9 +2635,11 @@ void launch_scatter_fused(hipStream_t s, const DeviceState &d, CellGrid &g, Stat
 void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const double g3[3], Status *st,
                        int sum_roles, double dt, double h, int zero_f, CellGrid &g)
 {
-    const unsigned nb = d.n > 0 ? blocks_for(d.n, NSLOT) : 1;
+    // the particles' workgroups, then one that cuts the next sweep's XCD runs
+    const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + 1;
     hipLaunchKernelGGL(k_update_bin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g3[0], g3[1], g3[2], st, sum_roles, dt,
-                       h, zero_f, grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank);
+                       h, zero_f, grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank, g.ncell, g.cell_start,
+                       g.cell_work, g.order);
 }
 
 void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double dt, double h, int zero_f,
@@ -2603,8 +2717,8 @@ void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     a.owners = owners;
     a.order = g.order;
     a.cell_work = g.cell_work;
-    // one workgroup per cell, rounded up to a multiple of the 8 XCDs
-    const dim3 gr((unsigned)(8 * ((g.ncell + 7) / 8))), blk(CELL_WAVES * WAVE);
+    // the 8 XCDs' runs at their capacity (the workgroups past a run's length exit at once)
+    const dim3 gr((unsigned)(8 * cell_run_cap(g.ncell))), blk(CELL_WAVES * WAVE);
 #define TB3_CELLS(T, G, E) hipLaunchKernelGGL((k_p3bca_cells<T, G, E>), gr, blk, 0, s, a)
     if (mixed) {
         if (grad) {
