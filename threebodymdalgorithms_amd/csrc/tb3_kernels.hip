@@ -931,13 +931,20 @@ __device__ __forceinline__ bool within_cutoff(const CutDev &a, const SysGrid &sg
     return !(sqr_dist_periodic(ax, ay, az, bx, by, bz, sg.L[0], sg.L[1], sg.L[2]) > a.c2);
 }
 
-// Owner slot for block b: blocks are spread round-robin over the 8 XCDs, so
-// block b runs on XCD b % 8; giving each XCD a contiguous stretch of the
-// cell-ordered particles keeps a cell neighbourhood in one XCD's L2.
-__device__ __forceinline__ int owner_slot(int b, int n)
+// Owner slot for block b (one wave per owner): blocks are spread
+// round-robin over the 8 XCDs, so block b runs on XCD b % 8.  Each XCD takes
+// every 8th stretch of OWNER_STRETCH consecutive cell-ordered particles:
+// a stretch keeps a cell neighbourhood in one XCD's L2, and interleaving the
+// stretches spreads the density's variations (and so the work) evenly over
+// the XCDs, where one contiguous eighth of the particles per XCD left the
+// XCDs' work up to ~12% apart (the exact sweep ran 5% faster interleaved).
+// Slots from n up to the grid's end are empty (owner_grid).
+constexpr int OWNER_STRETCH = 256;
+
+__device__ __forceinline__ int owner_slot(int b)
 {
-    const int chunk = (n + 7) >> 3;
-    return (b & 7) * chunk + (b >> 3);
+    const int k = b >> 3, r = b & 7;
+    return ((k / OWNER_STRETCH) * 8 + r) * OWNER_STRETCH + k % OWNER_STRETCH;
 }
 
 // ---------------------------------------------------------------------------
@@ -973,7 +980,7 @@ __global__ __launch_bounds__(WAVE) void k_p3bca_fast(CutDev a)
 
     // a halted step-graph replay leaves the step to the host
     if (a.st->halt) return;
-    const int slot = owner_slot(blockIdx.x, a.n);
+    const int slot = owner_slot((int)blockIdx.x);
     if (slot >= a.n) return;
     const int p = a.cell_idx[slot];
     const int lane = lane_id();
@@ -2153,7 +2160,7 @@ __global__ __launch_bounds__(WAVE, 5) void k_p3bca_exact(CutDev a)
     __shared__ int sh_start[32], sh_pre[32];
     // a halted step-graph replay leaves the step to the host
     if (a.st->halt) return;
-    const int slot = owner_slot((int)blockIdx.x, a.n);
+    const int slot = owner_slot((int)blockIdx.x);
     if (slot >= a.n) return;
     const int p = a.cell_idx[slot];
     const int lane = lane_id();
@@ -2528,7 +2535,7 @@ CutDev cut_dev(const DeviceState &d, const CellGrid &g, const CutoffSpec &c, dou
     return a;
 }
 
-unsigned owner_grid(int n) { return 8u * (unsigned)((n + 7) / 8); }
+unsigned owner_grid(int n) { return 8u * OWNER_STRETCH * (unsigned)((n + 8 * OWNER_STRETCH - 1) / (8 * OWNER_STRETCH)); }
 
 }  // namespace
 
