@@ -219,6 +219,21 @@ tb3_status tb3_get_tuning(const tb3_ctx *ctx, double *cell_ratio, int *blocks_pe
  * counterpart in the reference. */
 tb3_status tb3_set_tuning(tb3_ctx *ctx, double cell_ratio, int dense_waves);
 
+/* The slotted cell layout of tb3_run's replayed steps (fast cutoff sweep on
+ * one system, grids of up to 8192 cells, no dense-owner kernel): each cell
+ * keeps `slots` slots that the binning pass fills directly, so that a step
+ * runs no scatter pass between the binning and the sweep.  slots = -1 picks
+ * three times the mean cell count (at least 32), 0 turns the layout off,
+ * 1 .. 256 sets it; TB3_EINVAL otherwise.  A step in which some cell gets
+ * more particles than slots is finished on the contiguous layout, which the
+ * rest of the run (and later runs on the same grid) then keeps.
+ * tb3_get_cell_slots reports the setting and the slots per cell that the
+ * last captured step graph used (0: contiguous, or a cell has outgrown its
+ * slots since).  No counterpart in the reference, whose cell lists are host
+ * containers. */
+tb3_status tb3_set_cell_slots(tb3_ctx *ctx, int slots);
+tb3_status tb3_get_cell_slots(const tb3_ctx *ctx, int *requested, int *in_use);
+
 /* Per-phase timing with HIP events recorded on the context's stream around
  * each phase of tb3_step, tb3_run (which runs its steps one pass at a time
  * while profiling is on), tb3_compute_forces and the device-pointer energy

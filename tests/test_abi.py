@@ -59,6 +59,8 @@ def test_null_context_is_einval():
     assert lib.tb3_get_tuning(None, None, None, None) == _abi.EINVAL
     assert lib.tb3_force_scale(None, None) == _abi.EINVAL
     assert lib.tb3_set_tuning(None, 1.0, 8) == _abi.EINVAL
+    assert lib.tb3_set_cell_slots(None, -1) == _abi.EINVAL
+    assert lib.tb3_get_cell_slots(None, None, None) == _abi.EINVAL
 
 
 @pytest.mark.skipif(has_gpu(), reason="checks the no-device error path")

@@ -150,6 +150,26 @@ def test_c1_fast_sweeps_initial_state(eng, oracle):
             assert e["max_rel_floored"] <= 1e-6 and e["rms_rel"] <= 1e-5
 
 
+def test_c1_fast_run_counts_match_exact(eng):
+    """C1's replayed FP64 steps (slotted cell layout: the binning fills each
+    cell's slots, no scatter pass) count the same triplets per step as the
+    exact path, which is the reference bit for bit, and as replays on the
+    contiguous layout."""
+    P = c1_particles()
+    Z = np.zeros_like(P)
+    res = {}
+    for prec, slots in (("exact", -1), ("fp64", -1), ("fp64", 0)):
+        eng.set_cell_slots(slots)
+        eng.set_particles(P, Z, Z, np.ones(len(P)))
+        eng.configure("p3bca", cutoff=2.0, precision=prec)
+        say(f"C1 run of 4 steps {prec}, slots {slots}")
+        res[(prec, slots)] = [int(c) for c in eng.run(4, 0.001, (0.0, -9.81, 0.0))]
+        assert (eng.get_cell_slots()[1] > 0) == (prec == "fp64" and slots == -1)
+    eng.set_cell_slots(-1)
+    say(f"  counts {res}")
+    assert res[("fp64", -1)] == res[("exact", -1)] == res[("fp64", 0)]
+
+
 def test_c0_auta_2048_one_step(eng, oracle):
     P = generators.csv_rounded(generators.uniform(2048, 10.0, SEED))
     n = len(P)

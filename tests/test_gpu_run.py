@@ -7,7 +7,8 @@ and host synchronisation between the replays:
   * runs in which a sweep overflows part way (two clusters that merge):
     the halted replays leave the state alone and the host finishes the step,
     so the result still equals the one-step-at-a-time run (bit for bit on the
-    exact path), also where the dense-owner kernel is the one that overflows.
+    exact path), also where the dense-owner kernel is the one that overflows,
+    and where a cell outgrows the slots of the slotted cell layout.
 """
 import sys
 
@@ -82,6 +83,8 @@ def test_run_equals_steps_p3bca(eng, precision):
     # the captured graph is reused by a second run on a new state
     got2, cgot2 = state_after(eng, P, V, "p3bca", 2.0, precision, [("run", 12)])
     assert cgot2[:ncmp] == cref[:ncmp]
+    # the fast paths' replays bin into the slotted cell layout, the exact path's do not
+    assert (eng.get_cell_slots()[1] > 0) == (not exact)
     if exact:
         assert_same(got2, cgot2, ref, cref, True)
     else:
@@ -131,6 +134,60 @@ def test_run_with_overflow_midway(precision):
         got, cgot = state_after(e2, P, V, "p3bca", 2.0, precision, [("run", 9)], L=L, nu=1e-9)
     assert max(cref) > 3 * cref[0]  # the neighbourhoods did grow
     assert_same(got, cgot, ref, cref, precision == "exact")
+
+
+def slot_lattice():
+    """8 particles in each of the 6^3 cells (width 2.1: a 12.6^3 box, cutoff
+    2) : the particles at x offset 1.55 of the cells with even x index drift
+    at 100 / s, so that from step 5 on those cells' right neighbours hold 12."""
+    rng = np.random.default_rng(23)
+    off = np.array([[a, b, c] for a in (0.5, 1.55) for b in (0.5, 1.5) for c in (0.5, 1.5)])
+    cells = np.array([[i, j, k] for i in range(6) for j in range(6) for k in range(6)], dtype=float)
+    P = (2.1 * cells[:, None, :] + off[None, :, :]).reshape(-1, 3)
+    P += rng.uniform(-0.02, 0.02, P.shape)
+    V = np.zeros_like(P)
+    drift = (np.repeat(cells[:, 0], 8) % 2 == 0) & (np.tile(off[:, 0], len(cells)) == 1.55)
+    V[drift, 0] = 100.0
+    return P, V
+
+
+def test_run_slots_overflow_midway(eng):
+    """With 8 slots per cell the steps replay on the slotted layout until a
+    cell receives more particles than slots (step 5); that step and the rest
+    of the run then take the contiguous layout, and the run still equals the
+    one-step-at-a-time run (counts equal, state within 1e-10 of each array's
+    scale: the FP64 force sums are atomic)."""
+    P, V = slot_lattice()
+    L = np.array([12.6, 12.6, 12.6])
+    say("run with a slot overflow at step 5 fp64")
+    ref, cref = state_after(eng, P, V, "p3bca", 2.0, "fp64", [("step", 1)] * 9, L=L)
+    eng.set_cell_slots(8)
+    try:
+        state_after(eng, P, V, "p3bca", 2.0, "fp64", [("run", 4)], L=L)
+        assert eng.get_cell_slots() == (8, 8)  # four steps on 8 slots per cell
+        got, cgot = state_after(eng, P, V, "p3bca", 2.0, "fp64", [("run", 9)], L=L)
+        assert eng.get_cell_slots() == (8, 0)  # a cell outgrew its slots
+        assert_same(got, cgot, ref, cref, False, tol=1e-10)
+    finally:
+        eng.set_cell_slots(-1)
+
+
+def test_cell_slots_setting(eng):
+    """Slots off, automatic or set give the same counts; values outside
+    -1 .. 256 are refused."""
+    P = uniform_particles(3000, (1500.0) ** (1 / 3), 44)
+    V = np.zeros_like(P)
+    counts = {}
+    for slots in (0, -1, 64):
+        eng.set_cell_slots(slots)
+        _, counts[slots] = state_after(eng, P, V, "p3bca", 2.0, "fp64", [("run", 3)])
+        req, use = eng.get_cell_slots()
+        assert req == slots and (use == 0) == (slots == 0) and (slots <= 0 or use == slots)
+    eng.set_cell_slots(-1)
+    assert counts[0] == counts[-1] == counts[64]
+    for bad in (-2, 257):
+        with pytest.raises(tb3.TB3Error):
+            eng.set_cell_slots(bad)
 
 
 def lattice_cluster(n, h, centre, rng):

@@ -67,6 +67,8 @@ SIGNATURES = {
     "tb3_force_scale": (ctypes.c_int, [_ctxp, _dp]),
     "tb3_get_tuning": (ctypes.c_int, [_ctxp, _dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "tb3_set_tuning": (ctypes.c_int, [_ctxp, ctypes.c_double, ctypes.c_int]),
+    "tb3_set_cell_slots": (ctypes.c_int, [_ctxp, ctypes.c_int]),
+    "tb3_get_cell_slots": (ctypes.c_int, [_ctxp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "tb3_set_profiling": (ctypes.c_int, [_ctxp, ctypes.c_int]),
     "tb3_get_profile": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64]),
     "tb3_selftest_fp64": (ctypes.c_int, [_ctxp, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]),

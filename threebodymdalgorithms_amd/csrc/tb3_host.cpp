@@ -155,6 +155,14 @@ struct tb3_ctx {
     int list_cap = 128;    // LDS neighbour-list capacity per wave (power of two)
     bool cells_ok = true;  // the cell-per-workgroup kernel staged every neighbourhood so far
     bool dense_on = false;  // the dense-owner kernel follows the cell kernel (once some owner needed it)
+    // the slotted cell layout of replayed steps (CellGrid::pad_cap): slots
+    // per cell asked for (-1: three times the mean cell count, 0: off), and
+    // whether a cell outgrew its slots in this particle system's grid (the
+    // layout then stays off until the grid changes)
+    int slots_req = -1;
+    bool slots_failed = false;
+    long long slot_buf_cap = 0;  // slot entries allocated
+    int slot_cnt_cap = 0;        // slotted count entries allocated (2 ncell)
     int dense_grid = 1;    // persistent grid of the dense-owner kernel (workgroups that fit the GPU)
     int dense_waves = 8;   // waves per owner in the dense-owner kernel (tuned: 4, 8 or 16)
     int ncu = 1;           // compute units of the device
@@ -186,6 +194,7 @@ struct tb3_ctx {
     int graph_cap = -1;
     bool graph_cells_ok = false;
     bool graph_dense_on = false;
+    int graph_slots = 0;  // the slots per cell the capture used (0: contiguous layout)
     unsigned long long *d_counts = nullptr;
     unsigned long long *h_counts = nullptr;  // pinned, counts_cap entries
     long long counts_cap = 0;
@@ -355,6 +364,7 @@ void configure_grid(tb3_ctx *c)
         HIPCHK(hipMemset(g.cell_work, 0, (size_t)g.ncell * sizeof(unsigned)));
     c->cells_ok = true;  // a new grid gets a new chance at the cell kernel
     c->dense_on = false;
+    c->slots_failed = false;
     if (!same) ++c->epoch;
 }
 
@@ -409,6 +419,44 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
     }
 }
 
+// Slots per cell of the slotted layout for n particles: three times the
+// mean cell count (at least 32, a multiple of 16) unless set by
+// tb3_set_cell_slots; 0 where the layout does not apply (off, failed, a
+// replica batch, or cells too full to stage).
+int slots_for(const tb3_ctx *c, int n)
+{
+    const CellGrid &g = c->grid;
+    if (c->slots_req == 0 || c->slots_failed || g.nrep > 1 || g.ncell <= 0 || n < 3) return 0;
+    if (c->slots_req > 0) return c->slots_req;
+    const int mean = (int)((n + (long long)g.ncell - 1) / g.ncell);
+    const int cap = std::max(32, (3 * mean + 15) / 16 * 16);
+    return cap <= 256 ? cap : 0;
+}
+
+// Sets grid.pad_cap for the current particle count and allocates the slot
+// buffers it needs.
+void ensure_slot_buffers(tb3_ctx *c, int n)
+{
+    CellGrid &g = c->grid;
+    g.pad_cap = slots_for(c, n);
+    if (g.pad_cap == 0) return;
+    const long long need = (long long)g.ncell * g.pad_cap;
+    if (need > c->slot_buf_cap) {
+        ++c->epoch;
+        dev_alloc(g.pad_idx, need);
+        dev_alloc(g.pad_x, need);
+        dev_alloc(g.pad_y, need);
+        dev_alloc(g.pad_z, need);
+        c->slot_buf_cap = need;
+    }
+    if (2 * g.ncell > c->slot_cnt_cap) {
+        ++c->epoch;
+        dev_alloc(g.pad_cnt, 2LL * g.ncell);
+        dev_alloc(g.pad_state, 2);
+        c->slot_cnt_cap = 2 * g.ncell;
+    }
+}
+
 void ensure_triples(tb3_ctx *c, long long n)
 {
     if (c->triples_n == n) return;
@@ -446,7 +494,9 @@ int next_pow2(int v)
 //   energy : E into the status block
 // The sweep of a pass.  With `tail` it ends with the role sum (exact path)
 // and the status fold; an integrating pass leaves both to launch_tail.
-void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool energy, bool tail)
+// `slotted`: the cell kernel sweeps the slotted layout (replayed steps).
+void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool energy, bool tail,
+                  bool slotted = false)
 {
     const bool cut = uses_cutoff(c);
     const bool exact = c->precision == TB3_FP64_EXACT;
@@ -464,7 +514,7 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
             const int slot = (exact && grad) ? 1 : 0;  // the exact pass owns count[0]
             if (cut && c->cells_ok) {
                 launch_p3bca_cells(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
-                                   energy ? 1 : 0, c->d_owners, c->dense_on ? 1 : 0);
+                                   energy ? 1 : 0, c->d_owners, c->dense_on ? 1 : 0, slotted ? 1 : 0);
                 if (c->dense_on)
                     launch_p3bca_dense(s, d, c->grid, cs, c->nu, mixed, fast_grad ? 1 : 0, c->d_status, slot,
                                        energy ? 1 : 0, c->d_owners, c->dense_grid, c->dense_waves);
@@ -487,11 +537,13 @@ void launch_sweep(tb3_ctx *c, hipStream_t s, DeviceState &d, bool grad, bool ene
 // The end of an integrating pass after launch_sweep(..., tail = false): one
 // kernel for the role sum, the status fold and the update (and, in a step
 // graph, the count record and the status clear for the next replay).
-void launch_tail(tb3_ctx *c, hipStream_t s, DeviceState &d, double dt, const double *g, bool grad, bool record)
+void launch_tail(tb3_ctx *c, hipStream_t s, DeviceState &d, double dt, const double *g, bool grad, bool record,
+                 bool slotted = false)
 {
     const bool sum_roles = c->precision == TB3_FP64_EXACT && grad && d.n >= 3;
     const bool cells = uses_cutoff(c) && d.n >= 3 && c->grid.order;
-    launch_update_fin(s, d, dt * 0.5, g, c->d_status, sum_roles ? 1 : 0, record ? 1 : 0, cells ? &c->grid : nullptr);
+    launch_update_fin(s, d, dt * 0.5, g, c->d_status, sum_roles ? 1 : 0, record ? 1 : 0, cells ? &c->grid : nullptr,
+                      slotted ? 1 : 0);
     HIPCHK(hipGetLastError());
 }
 
@@ -621,6 +673,15 @@ bool graph_ok(const tb3_ctx *c)
 // count scan runs inside the scatter kernel.
 bool fused_steps(const tb3_ctx *c) { return uses_cutoff(c) && c->state.d.n >= 3 && cells_fused_scan(c->grid); }
 
+// Fused replays of the cell kernel alone (fast sweep, no dense-owner
+// kernel, one system) run on the slotted layout: the binning writes each
+// cell's slots itself and no scatter kernel runs.  ensure_slot_buffers sets
+// grid.pad_cap first.
+bool slotted_steps(const tb3_ctx *c)
+{
+    return fused_steps(c) && c->precision != TB3_FP64_EXACT && c->cells_ok && !c->dense_on && c->grid.pad_cap > 0;
+}
+
 template <class F>
 hipGraphExec_t capture(hipStream_t s, F &&body)
 {
@@ -644,9 +705,11 @@ hipGraphExec_t capture(hipStream_t s, F &&body)
 void capture_step_graph(tb3_ctx *c, double dt, const double *g)
 {
     const bool fused = fused_steps(c);
+    const int slots = slotted_steps(c) ? c->grid.pad_cap : 0;
     if (c->step_exec && c->graph_epoch == c->epoch && c->graph_dt == dt && c->graph_g[0] == g[0] &&
         c->graph_g[1] == g[1] && c->graph_g[2] == g[2] && c->graph_cap == c->list_cap &&
-        c->graph_cells_ok == c->cells_ok && c->graph_dense_on == c->dense_on && c->graph_fused == fused)
+        c->graph_cells_ok == c->cells_ok && c->graph_dense_on == c->dense_on && c->graph_fused == fused &&
+        c->graph_slots == slots)
         return;
     if (c->step_exec) {
         (void)hipGraphExecDestroy(c->step_exec);
@@ -661,17 +724,18 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
     DeviceState &d = c->state.d;
     if (fused) {
         const bool exact = c->precision == TB3_FP64_EXACT;
+        const bool slotted = slots > 0;  // then the binning fills the slots and no scatter runs
         auto mid = [&] {
-            launch_scatter_fused(s, d, c->grid, c->d_status);
-            launch_sweep(c, s, d, true, false, false);
+            if (!slotted) launch_scatter_fused(s, d, c->grid, c->d_status);
+            launch_sweep(c, s, d, true, false, false, slotted);
             launch_update_bin(s, d, dt * 0.5, g, c->d_status, exact ? 1 : 0, dt, dt * dt * 0.5, exact ? 0 : 1,
-                              c->grid);
+                              c->grid, slotted ? 1 : 0);
         };
         c->step_exec_mid = capture(s, mid);
         c->step_exec = capture(s, [&] {
-            launch_scatter_fused(s, d, c->grid, c->d_status);
-            launch_sweep(c, s, d, true, false, false);
-            launch_tail(c, s, d, dt, g, true, true);
+            if (!slotted) launch_scatter_fused(s, d, c->grid, c->d_status);
+            launch_sweep(c, s, d, true, false, false, slotted);
+            launch_tail(c, s, d, dt, g, true, true, slotted);
         });
     } else {
         c->step_exec = capture(s, [&] {
@@ -681,6 +745,7 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
         });
     }
     c->graph_fused = fused;
+    c->graph_slots = slots;
     c->graph_epoch = c->epoch;
     c->graph_dt = dt;
     for (int k = 0; k < 3; ++k) c->graph_g[k] = g[k];
@@ -724,15 +789,23 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
             return;
         }
         // what a step would allocate is allocated before the capture
-        if (uses_cutoff(c) && d.n >= 3) ensure_grid_buffers(c, d.n);
+        if (uses_cutoff(c) && d.n >= 3) {
+            ensure_grid_buffers(c, d.n);
+            ensure_slot_buffers(c, d.n);
+        }
         if (!uses_cutoff(c) && d.n >= 3 && c->precision != TB3_FP64_EXACT) ensure_triples(c, d.n);
         capture_step_graph(c, dt, g);
         launch_begin_steps(s, c->d_status, c->d_counts);
         if (c->graph_fused) {
             // the first step's predictor and binning; each step_exec_mid
             // replay ends with the next step's
-            launch_bin_ranked(s, d, 1, dt, dt * dt * 0.5, c->precision == TB3_FP64_EXACT ? 0 : 1, c->grid,
-                              c->d_status);
+            if (c->graph_slots > 0) {
+                launch_slots_begin(s, c->grid, c->d_status);
+                launch_bin_slotted(s, d, dt, dt * dt * 0.5, 1, c->grid, c->d_status);
+            } else {
+                launch_bin_ranked(s, d, 1, dt, dt * dt * 0.5, c->precision == TB3_FP64_EXACT ? 0 : 1, c->grid,
+                                  c->d_status);
+            }
             for (int k = done; k + 1 < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec_mid, s));
             HIPCHK(hipGraphLaunch(c->step_exec, s));
         } else {
@@ -753,6 +826,18 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
         if (done < steps) {
             // step `done` overflowed: its predictor ran, its update did not
             if (!c->h_status->flags[2]) throw Fail(TB3_EHIP, "replayed steps halted without an overflow");
+            if (c->graph_fused && c->graph_slots > 0) {
+                // that step was binned into the slots: the host's sweeps
+                // below need its contiguous cell list (the counters of the
+                // contiguous binning are at zero, untouched by the slotted
+                // steps), built once the halt is cleared (the scatter
+                // kernel skips halted replays; the status read above keeps
+                // what the host needs); a cell that outgrew its slots turns
+                // the layout off
+                if (c->h_status->pad_over) c->slots_failed = true;
+                launch_reset_status(s, c->d_status, 1);
+                launch_cells_build(s, d, 0, dt, dt * dt * 0.5, 0, c->grid, c->d_status, c->scan_tmp);
+            }
             finish_overflowed(c, s, d, true, dt, g, true, false);
             if (counts) step_counts(c, counts + (size_t)done * per);
             ++done;
@@ -883,6 +968,15 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.cell_rank);
     dev_free(c->grid.scatter_ticket);
     dev_free(c->grid.cell_idx);
+    dev_free(c->grid.xs);
+    dev_free(c->grid.ys);
+    dev_free(c->grid.zs);
+    dev_free(c->grid.pad_cnt);
+    dev_free(c->grid.pad_state);
+    dev_free(c->grid.pad_idx);
+    dev_free(c->grid.pad_x);
+    dev_free(c->grid.pad_y);
+    dev_free(c->grid.pad_z);
     dev_free(c->scan_tmp);
     dev_free(c->triples);
     dev_free(c->d_status);
@@ -1594,6 +1688,25 @@ tb3_status tb3_set_tuning(tb3_ctx *c, double cell_ratio, int dense_waves)
             if (c->n > 0 && uses_cutoff(c)) configure_grid(c);
         }
     });
+}
+
+tb3_status tb3_set_cell_slots(tb3_ctx *c, int slots)
+{
+    if (!c) return TB3_EINVAL;
+    if (!(slots == -1 || slots == 0 || (slots >= 1 && slots <= 256)))
+        return fail(c, TB3_EINVAL, "slots must be -1 (automatic), 0 (off) or 1 .. 256");
+    c->slots_req = slots;
+    c->slots_failed = false;
+    ++c->epoch;
+    return TB3_OK;
+}
+
+tb3_status tb3_get_cell_slots(const tb3_ctx *c, int *requested, int *in_use)
+{
+    if (!c) return TB3_EINVAL;
+    if (requested) *requested = c->slots_req;
+    if (in_use) *in_use = c->graph_fused && !c->slots_failed ? c->graph_slots : 0;
+    return TB3_OK;
 }
 
 tb3_status tb3_set_profiling(tb3_ctx *c, int enable)

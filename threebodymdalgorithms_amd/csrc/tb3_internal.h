@@ -59,6 +59,20 @@ struct CellGrid {
     double *rlo = nullptr, *rL = nullptr;      // device, 3 nrep
     unsigned long long *rep_count = nullptr;   // device, nrep: accepted triplets per replica
     double *rep_energy = nullptr;              // device, nrep: energy per replica
+
+    // Slotted cell layout of replayed steps (one system, fast sweep, no
+    // dense-owner kernel): cell c's particles sit at slots [c pad_cap,
+    // c pad_cap + count), written by the binning pass itself, so that no
+    // scatter pass runs between the binning and the sweep.  The counts
+    // alternate between two arrays by the parity of an epoch that each
+    // binning pass advances: the sweep at epoch e reads pad_cnt[e & 1] and
+    // clears its own cell's entry of pad_cnt[(e + 1) & 1], which the next
+    // binning fills.  A count past pad_cap sets Status::pad_over.
+    int pad_cap = 0;                 // slots per cell; 0: no slotted buffers
+    int *pad_cnt = nullptr;          // 2 ncell
+    unsigned *pad_state = nullptr;   // [0] epoch, [1] finished workgroups of a binning pass (returns to 0)
+    int *pad_idx = nullptr;          // ncell pad_cap particle indices
+    double *pad_x = nullptr, *pad_y = nullptr, *pad_z = nullptr;  // ncell pad_cap positions
 };
 
 struct CutoffSpec {
@@ -102,6 +116,11 @@ struct Status {
     int nrep;
     unsigned long long *rep_count;
     double *rep_energy;
+    // a cell of the slotted layout received more particles than it has
+    // slots (the sweep then fails the pass); cleared only by
+    // launch_slots_begin, never by a status clear, since the binning that
+    // sets it runs beside the clear of the replayed step's last kernel
+    int pad_over;
 };
 
 // ---- launchers (all asynchronous on `s`) ----
@@ -124,7 +143,7 @@ void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h);
 // With `cells` (a cell list this pass swept), one more workgroup cuts the
 // next sweep's XCD runs from the per-cell work.
 void launch_update_fin(hipStream_t s, const DeviceState &d, double hdt, const double g[3], Status *st, int sum_roles,
-                       int record, const CellGrid *cells);
+                       int record, const CellGrid *cells, int slotted = 0);
 void launch_sum_roles(hipStream_t s, const DeviceState &d);
 void launch_zero(hipStream_t s, double *p, long long count);
 
@@ -150,7 +169,16 @@ void launch_scatter_fused(hipStream_t s, const DeviceState &d, CellGrid &g, Stat
 // grids): launch_update_fin's work with record, then the next step's
 // predictor, force clear (zero_f) and binning.
 void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const double g3[3], Status *st,
-                       int sum_roles, double dt, double h, int zero_f, CellGrid &g);
+                       int sum_roles, double dt, double h, int zero_f, CellGrid &g, int slotted = 0);
+// The slotted layout (CellGrid::pad_cap): launch_slots_begin clears both
+// count arrays, the epoch, the ticket and Status::pad_over before a run of
+// replayed steps; launch_bin_slotted is that run's first predictor (+ force
+// clear) and binning into the slots, with the cell order of the first sweep.
+// launch_update_bin and launch_update_fin with `slotted` read and fill the
+// slotted counts instead (and order the next sweep's cells themselves).
+void launch_slots_begin(hipStream_t s, CellGrid &g, Status *st);
+void launch_bin_slotted(hipStream_t s, const DeviceState &d, double dt, double h, int zero_f, CellGrid &g,
+                        Status *st);
 
 // Cutoff sweeps (P3BCA).  One wave per particle builds that particle's
 // neighbour list in LDS (capacity `cap`) from the cell list; a list longer
@@ -171,8 +199,10 @@ void launch_p3bca_exact(hipStream_t s, const DeviceState &d, const CellGrid &g, 
 // st->want_dense and st->flags[2] (the caller re-runs the pass with the
 // dense-owner kernel).  Non-finite positions set st->flags[2] and [3]; the
 // caller then re-runs the sweep with launch_p3bca_fast.
+// `slotted`: sweep the slotted layout (needs dense_on == 0).
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int dense_on);
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int dense_on,
+                        int slotted = 0);
 // Workgroups of launch_p3bca_cells that fit on one CU at once.
 int p3bca_cells_blocks_per_cu();
 // The cell kernel hands owners whose neighbourhood does not fit its LDS

@@ -236,26 +236,27 @@ __global__ void k_predict(DeviceState d, double dt, double h)
 // the longest cells first: with under two workgroups per slot, the cells that
 // start last otherwise decide when the sweep ends.  Each cell's work is its
 // triplet count in the previous sweep (cell_work: most particles move little
-// in a step), or 16 n^2 for a cell that has none yet (n particles; start(c)
-// is cell c's first slot).
+// in a step), or 16 n^2 for a cell that has none yet (n = count(c)
+// particles).
 //
 // The cuts: cell c joins run floor(8 (W_c + w_c / 2) / W), W_c being the
 // work of the cells before it and W the total, which gives contiguous runs
 // of about W / 8 each, so that the XCDs finish together (runs of equal cell
 // counts left one XCD with 12% more work than the mean on a uniform
 // system).  Should a run exceed the capacity, every run takes ncell / 8 cells
-// instead.  One workgroup of 256 threads; the 9 cuts go to out.  LDS: fpart
-// (4 floats), cut (9 ints).
-template <class Start>
-__device__ void compute_cuts(int ncell, const Start &start, const unsigned *__restrict__ cell_work, int *out,
-                             float *fpart, int *cut)
+// instead.  One workgroup of 256 threads; the 9 cuts go to out (with
+// write_out) and stay in cut[] for order_runs.  LDS: fpart (4 floats), cut
+// (9 ints).
+template <class Count>
+__device__ void compute_cuts(int ncell, const Count &count, const unsigned *__restrict__ cell_work, int *out,
+                             float *fpart, int *cut, bool write_out = true)
 {
     const int t = threadIdx.x;
     const int run = (ncell + 7) >> 3, cap = cell_run_cap(ncell);
     auto work = [&](int c) {
         const unsigned w = cell_work[c];
         if (w) return (float)w;
-        const float n = (float)(start(c + 1) - start(c));
+        const float n = (float)count(c);
         return 16.0f * n * n;
     };
     // each thread sums a chunk of cells (their work kept in registers, up to
@@ -301,7 +302,8 @@ __device__ void compute_cuts(int ncell, const Start &start, const unsigned *__re
         cut[0] = 0;
         for (int r = 7; r >= 1; --r) cut[r] = min(cut[r], cut[r + 1]);
         for (int r = 0; r < 8 && ok; ++r) ok = cut[r + 1] - cut[r] <= cap;
-        for (int r = 0; r <= 8; ++r) out[r] = ok ? cut[r] : min(r * run, ncell);
+        if (write_out)
+            for (int r = 0; r <= 8; ++r) out[r] = ok ? cut[r] : min(r * run, ncell);
     }
 }
 
@@ -312,8 +314,8 @@ __device__ void compute_cuts(int ncell, const Start &start, const unsigned *__re
 // (overlapping that load with its own), if they fit this grid; else each
 // run takes ncell / 8 cells.  256 threads; LDS: hist (256 ints), cut (9
 // ints).
-template <class Start>
-__device__ void order_runs(int ncell, const Start &start, const unsigned *__restrict__ cell_work,
+template <class Count>
+__device__ void order_runs(int ncell, const Count &count, const unsigned *__restrict__ cell_work,
                            int *__restrict__ order, int *hist, int *cut, int r0, int rstep)
 {
     if (r0 >= 8) return;
@@ -321,7 +323,7 @@ __device__ void order_runs(int ncell, const Start &start, const unsigned *__rest
     const int run = (ncell + 7) >> 3, cap = cell_run_cap(ncell);
     auto bin = [&](int c) {
         const unsigned w = cell_work[c];
-        const float n = (float)(start(c + 1) - start(c));
+        const float n = (float)count(c);
         const float k = w ? (float)w : 16.0f * n * n;
         return 255 - min(255, (int)(16.0f * __log2f(1.0f + k)));
     };
@@ -355,6 +357,127 @@ __device__ void order_runs(int ncell, const Start &start, const unsigned *__rest
     }
 }
 
+// Grids of at most this many cells scan their counts inside the scatter
+// kernel (k_cell_scatter_fused) and may replay on the slotted layout.
+constexpr int FUSED_SCAN_CELLS = 8192;
+
+// compute_cuts and order_runs in one workgroup of 256 threads for run r,
+// on per-cell weights loaded once into LDS (w: ncell floats): the same cuts
+// (written to out when write_out) and the same order, with a single round
+// trip to memory and 256-bin scans done in the waves.  LDS besides w:
+// fpart (4 floats), cut (9 ints), hist (256 ints), ipart (4 ints).
+template <class Count>
+__device__ void cut_and_order(int ncell, const Count &count, const unsigned *__restrict__ cell_work, int *out,
+                              bool write_out, int *__restrict__ order, int r, float *w, float *fpart, int *cut,
+                              int *hist, int *ipart)
+{
+    const int t = threadIdx.x, lane = t & (WAVE - 1), wv = t >> 6;
+    const int run = (ncell + 7) >> 3, cap = cell_run_cap(ncell);
+    for (int c = t; c < ncell; c += 256) {
+        const unsigned k = cell_work[c];
+        const float n = (float)count(c);
+        w[c] = k ? (float)k : 16.0f * n * n;
+    }
+    if (t <= 8) cut[t] = ncell;
+    __syncthreads();
+    // the cuts: chunk sums, their scan, each cell's run (as compute_cuts)
+    const int per = (ncell + 255) >> 8;
+    const int c0 = min(t * per, ncell), c1 = min(c0 + per, ncell);
+    float s = 0.0f;
+    for (int c = c0; c < c1; ++c) s += w[c];
+    float incl = s;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const float u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    if (lane == WAVE - 1) fpart[wv] = incl;
+    __syncthreads();
+    float before = 0.0f;
+    for (int k = 0; k < wv; ++k) before += fpart[k];
+    const float total = (fpart[0] + fpart[1]) + (fpart[2] + fpart[3]);
+    if (total > 0.0f) {
+        float acc = before + incl - s;
+        for (int c = c0; c < c1; ++c) {
+            const float wc = w[c];
+            const int rr = min(7, max(0, (int)(8.0f * (acc + 0.5f * wc) / total)));
+            atomicMin(&cut[rr], c);
+            acc += wc;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        bool ok = total > 0.0f;
+        cut[0] = 0;
+        for (int k = 7; k >= 1; --k) cut[k] = min(cut[k], cut[k + 1]);
+        for (int k = 0; k < 8 && ok; ++k) ok = cut[k + 1] - cut[k] <= cap;
+        for (int k = 0; k <= 8; ++k) {
+            cut[k] = ok ? cut[k] : min(k * run, ncell);
+            if (write_out) out[k] = cut[k];
+        }
+    }
+    __syncthreads();
+    // run r heaviest first: a counting sort on bins of 1/16 octave of the work
+    auto bin = [&](int c) { return 255 - min(255, (int)(16.0f * __log2f(1.0f + w[c]))); };
+    const int lo = cut[r], hi = cut[r + 1];
+    hist[t] = 0;
+    __syncthreads();
+    for (int c = lo + t; c < hi; c += 256) atomicAdd(&hist[bin(c)], 1);
+    __syncthreads();
+    const int own = hist[t];
+    int v = own;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    if (lane == WAVE - 1) ipart[wv] = v;
+    __syncthreads();
+    for (int k = 0; k < wv; ++k) v += ipart[k];
+    hist[t] = v - own;
+    __syncthreads();
+    for (int c = lo + t; c < hi; c += 256) order[r * cap + atomicAdd(&hist[bin(c)], 1)] = c;
+    if (t == 0) order[8 * cap + r] = hi - lo;
+}
+
+// The slotted cell layout (CellGrid::pad_cap) as the kernels see it.
+struct SlotsDev {
+    int cap;          // slots per cell; 0: the contiguous layout is in use
+    int *cnt;         // 2 ncell counts, by epoch parity
+    unsigned *state;  // [0] epoch, [1] finished workgroups of a binning pass
+    int *idx;         // ncell cap particle indices
+    double *x, *y, *z;
+};
+
+// Particle p (position px, py, pz) into a slot of cell c, counted in cnt; a
+// cell past its slots raises Status::pad_over instead.
+__device__ __forceinline__ void bin_slotted(const SlotsDev &sl, int *cnt, int c, int p, double px, double py,
+                                            double pz, Status *st)
+{
+    const int r = atomicAdd(&cnt[c], 1);
+    if (r < sl.cap) {
+        const long long k = (long long)c * sl.cap + r;
+        sl.idx[k] = p;
+        sl.x[k] = px;
+        sl.y[k] = py;
+        sl.z[k] = pz;
+    } else {
+        atomicOr(&st->pad_over, 1);
+    }
+}
+
+// Each workgroup of a slotted binning pass takes a ticket once its binning
+// is done; the last one to arrive advances the epoch and leaves the ticket
+// at zero.  Every workgroup's read of the old epoch has completed before its
+// ticket (its value addressed the counts before the barrier), so no fence is
+// needed; the next kernel sees the new epoch.  (An agent-scope fence here
+// wrote back the XCD's L2 in every workgroup: +7 us per step.)
+__device__ __forceinline__ void slots_ticket(const SlotsDev &sl, unsigned e)
+{
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicInc(&sl.state[1], gridDim.x - 1) == gridDim.x - 1) sl.state[0] = e + 1;
+}
+
 // The last kernel of an integrating pass.  Workgroup 0 first folds the
 // sweep's slot sums into the status block (k_finalize); with `record` (a
 // replayed step graph) it then appends the step's count and, unless the
@@ -368,15 +491,23 @@ __device__ void order_runs(int ncell, const Start &start, const unsigned *__rest
 __global__ __launch_bounds__(NSLOT) void k_update_fin(DeviceState d, double hdt, double gx, double gy, double gz,
                                                      Status *st, int sum_roles, int record, int ncell,
                                                      const int *__restrict__ cell_start,
-                                                     const unsigned *__restrict__ cell_work, int *__restrict__ order)
+                                                     const unsigned *__restrict__ cell_work, int *__restrict__ order,
+                                                     SlotsDev sl)
 {
     // with a cell list (ncell > 0) the last workgroup cuts the next sweep's
-    // XCD runs from this sweep's per-cell work
+    // XCD runs from this sweep's per-cell work (and cell counts, contiguous
+    // or slotted)
     if (ncell > 0 && blockIdx.x == gridDim.x - 1) {
         __shared__ float fpart[4];
         __shared__ int cut[9];
-        compute_cuts(ncell, [&](int c) { return cell_start[c]; }, cell_work, order + 8 * cell_run_cap(ncell) + 8,
-                     fpart, cut);
+        int *out = order + 8 * cell_run_cap(ncell) + 8;
+        if (sl.cap > 0) {
+            const int *cur = sl.cnt + (sl.state[0] & 1u) * ncell;
+            compute_cuts(ncell, [&](int c) { return min(cur[c], sl.cap); }, cell_work, out, fpart, cut);
+        } else {
+            compute_cuts(ncell, [&](int c) { return cell_start[c + 1] - cell_start[c]; }, cell_work, out, fpart,
+                         cut);
+        }
         return;
     }
     if (blockIdx.x == 0) {
@@ -628,50 +759,65 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
     else atomicAdd(&cell_cnt[c], 1);
 }
 
-// The end of one replayed step fused with the start of the next (cutoff
-// algorithm, grids of at most FUSED_SCAN_CELLS cells): k_update_fin's fold,
-// count record, status clear and Particle::Update (structs.hpp:57-99), then,
-// in the same thread, the next step's predictor (structs.hpp:40-55) on the
-// updated velocity and acceleration, the force clear and the binning of
-// k_bin<true>.  The same operations on the same values as the two kernels,
-// one launch fewer per step.  The clear keeps flags[0] (a non-finite
-// position, which the binning may raise concurrently): it can only be set
-// here when the last sweep ran without overflow, i.e. with every position
-// finite, or on the per-owner and exact kernels, where it is sticky anyway.
-__global__ __launch_bounds__(NSLOT) void k_update_bin(DeviceState d, double hdt, double gx, double gy, double gz,
-                                                     Status *st, int sum_roles, double dt, double h, int zero_f,
-                                                     GridDev g, int *__restrict__ cell_of,
-                                                     int *__restrict__ cell_cnt, int *__restrict__ cell_rank,
-                                                     int ncell, const int *__restrict__ cell_start,
+// A run of replayed steps on the slotted layout starts with every count at
+// zero, epoch 0 and a clear overflow flag (a kernel, so that nothing here is
+// a memset).
+__global__ void k_slots_begin(int ncell, SlotsDev sl, Status *st)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int i = t; i < 2 * ncell; i += gridDim.x * blockDim.x) sl.cnt[i] = 0;
+    if (t == 0) {
+        sl.state[0] = 0u;
+        sl.state[1] = 0u;
+        st->pad_over = 0;
+    }
+}
+
+// The first step of a slotted run: predictor, force clear and binning of
+// k_bin<true>, into the slots of the current epoch; 8 more workgroups order
+// the first sweep's cells from the cuts the last integrating pass stored
+// (the counts fill beside them, so cells never swept weigh alike).
+__global__ __launch_bounds__(256) void k_bin_slotted(DeviceState d, double dt, double h, int zero_f, GridDev g,
+                                                     SlotsDev sl, Status *st, int ncell,
                                                      const unsigned *__restrict__ cell_work, int *__restrict__ order)
 {
-    // the last workgroup cuts the next sweep's XCD runs from this sweep's
-    // per-cell work (the cell starts are the ones this sweep used)
-    if (blockIdx.x == gridDim.x - 1) {
-        __shared__ float fpart[4];
-        __shared__ int cut[9];
-        compute_cuts(ncell, [&](int c) { return cell_start[c]; }, cell_work, order + 8 * cell_run_cap(ncell) + 8,
-                     fpart, cut);
+    const int nbin = (int)gridDim.x - 8;
+    if ((int)blockIdx.x >= nbin) {
+        __shared__ int hist[256], cut[9];
+        if (threadIdx.x <= 8) cut[threadIdx.x] = order[8 * cell_run_cap(ncell) + 8 + threadIdx.x];
+        order_runs(ncell, [](int) { return 0; }, cell_work, order, hist, cut, (int)blockIdx.x - nbin, 8);
         return;
     }
-    if (blockIdx.x == 0) {
-        fold_slots(st);
-        __syncthreads();  // thread 0's fold (and halt) is visible to the workgroup
-        if (!st->halt) {
-            const int t = threadIdx.x, nrep = st->nrep, ns = st->nstep;
-            if (nrep > 1) {
-                for (int r = t; r < nrep; r += NSLOT) st->counts[(long long)ns * nrep + r] = st->rep_count[r];
-            } else if (t == 0) {
-                st->counts[ns] = st->count[0];
-            }
-            __syncthreads();  // every read of this step's sums precedes the clear
-            clear_status(st, true, t, NSLOT);
-            if (t == 0) st->nstep = ns + 1;
-        }
-    }
-    if (st->flags[2]) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.n) return;
+    const double aX = d.ax[p];
+    const double px = predicted(d.x[p], d.vx[p], aX, dt, h);
+    const double py = predicted(d.y[p], d.vy[p], aX, dt, h);
+    const double pz = predicted(d.z[p], d.vz[p], aX, dt, h);
+    d.x[p] = px;
+    d.y[p] = py;
+    d.z[p] = pz;
+    if (zero_f) {
+        d.fx[p] = 0.0;
+        d.fy[p] = 0.0;
+        d.fz[p] = 0.0;
+    }
+    if (!finite3(px, py, pz)) atomicOr(&st->flags[0], 1);
+    const SysGrid sg = grid_of_particle(g, p);
+    int cx, cy, cz;
+    cell_of_pos(sg, px, py, pz, cx, cy, cz);
+    const int c = sg.cbase + (cz * sg.nc[1] + cy) * sg.nc[0] + cx;
+    bin_slotted(sl, sl.cnt + (sl.state[0] & 1u) * ncell, c, p, px, py, pz, st);
+}
+
+// One particle of k_update_bin: update, predictor, force clear, binning
+// (contiguous: cell and rank; slotted: into the next epoch's slots).
+__device__ __forceinline__ void update_bin_one(const DeviceState &d, double hdt, double gx, double gy, double gz,
+                                               Status *st, int sum_roles, double dt, double h, int zero_f,
+                                               const GridDev &g, int *__restrict__ cell_of,
+                                               int *__restrict__ cell_cnt, int *__restrict__ cell_rank,
+                                               const SlotsDev &sl, int *next_cnt, int p)
+{
     double fx, fy, fz;
     if (sum_roles) {
         const long long n = d.n;
@@ -712,8 +858,79 @@ __global__ __launch_bounds__(NSLOT) void k_update_bin(DeviceState d, double hdt,
     int cx, cy, cz;
     cell_of_pos(sg, px, py, pz, cx, cy, cz);
     const int c = sg.cbase + (cz * sg.nc[1] + cy) * sg.nc[0] + cx;
-    cell_of[p] = c;
-    cell_rank[p] = atomicAdd(&cell_cnt[c], 1);
+    if (sl.cap > 0) {
+        bin_slotted(sl, next_cnt, c, p, px, py, pz, st);
+    } else {
+        cell_of[p] = c;
+        cell_rank[p] = atomicAdd(&cell_cnt[c], 1);
+    }
+}
+
+// The end of one replayed step fused with the start of the next (cutoff
+// algorithm, grids of at most FUSED_SCAN_CELLS cells): k_update_fin's fold,
+// count record, status clear and Particle::Update (structs.hpp:57-99), then,
+// in the same thread, the next step's predictor (structs.hpp:40-55) on the
+// updated velocity and acceleration, the force clear and the binning of
+// k_bin<true>.  The same operations on the same values as the two kernels,
+// one launch fewer per step.  The clear keeps flags[0] (a non-finite
+// position, which the binning may raise concurrently): it can only be set
+// here when the last sweep ran without overflow, i.e. with every position
+// finite, or on the per-owner and exact kernels, where it is sticky anyway.
+__global__ __launch_bounds__(NSLOT) void k_update_bin(DeviceState d, double hdt, double gx, double gy, double gz,
+                                                     Status *st, int sum_roles, double dt, double h, int zero_f,
+                                                     GridDev g, int *__restrict__ cell_of,
+                                                     int *__restrict__ cell_cnt, int *__restrict__ cell_rank,
+                                                     int ncell, const int *__restrict__ cell_start,
+                                                     const unsigned *__restrict__ cell_work, int *__restrict__ order,
+                                                     SlotsDev sl)
+{
+    const bool slotted = sl.cap > 0;
+    const int nupd = (int)gridDim.x - (slotted ? 8 : 1);  // the particles' workgroups
+    if ((int)blockIdx.x >= nupd) {
+        __shared__ float fpart[4], wl[FUSED_SCAN_CELLS];
+        __shared__ int cut[9], hist[256], ipart[4];
+        int *out = order + 8 * cell_run_cap(ncell) + 8;
+        if (!slotted) {
+            // one workgroup cuts the next sweep's XCD runs from this sweep's
+            // per-cell work (the scatter kernel then orders them)
+            compute_cuts(ncell, [&](int c) { return cell_start[c + 1] - cell_start[c]; }, cell_work, out, fpart,
+                         cut);
+            return;
+        }
+        // slotted (no scatter kernel follows): 8 workgroups each cut the runs
+        // and order one of them for the next sweep, from this sweep's work
+        // and counts.  flags[2] is the same for the whole grid (only a
+        // halted replay has it, and then nothing here runs or takes a ticket).
+        if (st->flags[2]) return;
+        const unsigned e = sl.state[0];
+        const int *cur = sl.cnt + (e & 1u) * ncell;
+        auto count = [&](int c) { return min(cur[c], sl.cap); };
+        const int r = (int)blockIdx.x - nupd;
+        cut_and_order(ncell, count, cell_work, out, r == 0, order, r, wl, fpart, cut, hist, ipart);
+        slots_ticket(sl, e);
+        return;
+    }
+    if (blockIdx.x == 0) {
+        fold_slots(st);
+        __syncthreads();  // thread 0's fold (and halt) is visible to the workgroup
+        if (!st->halt) {
+            const int t = threadIdx.x, nrep = st->nrep, ns = st->nstep;
+            if (nrep > 1) {
+                for (int r = t; r < nrep; r += NSLOT) st->counts[(long long)ns * nrep + r] = st->rep_count[r];
+            } else if (t == 0) {
+                st->counts[ns] = st->count[0];
+            }
+            __syncthreads();  // every read of this step's sums precedes the clear
+            clear_status(st, true, t, NSLOT);
+            if (t == 0) st->nstep = ns + 1;
+        }
+    }
+    if (st->flags[2]) return;
+    const unsigned e = slotted ? sl.state[0] : 0u;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < d.n) update_bin_one(d, hdt, gx, gy, gz, st, sum_roles, dt, h, zero_f, g, cell_of, cell_cnt, cell_rank,
+                                sl, slotted ? sl.cnt + ((e + 1u) & 1u) * ncell : nullptr, p);
+    if (slotted) slots_ticket(sl, e);
 }
 
 // Places each particle in its cell's range (index and a cell-ordered copy of
@@ -729,7 +946,8 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
 {
     __shared__ int hist[256], cut[9];
     if (threadIdx.x <= 8) cut[threadIdx.x] = order[8 * cell_run_cap(ncell) + 8 + threadIdx.x];
-    order_runs(ncell, [&](int c) { return cell_start[c]; }, cell_work, order, hist, cut, blockIdx.x, gridDim.x);
+    order_runs(ncell, [&](int c) { return cell_start[c + 1] - cell_start[c]; }, cell_work, order, hist, cut,
+               blockIdx.x, gridDim.x);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int c = cell_of[p];
@@ -751,8 +969,6 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, const double *__res
 // XCD's run (order_runs), so the ordering runs beside the scatter instead of
 // after it.  Each scatter thread loads its particle before the scan, which
 // overlaps the two memory round trips.
-constexpr int FUSED_SCAN_CELLS = 8192;
-
 __global__ __launch_bounds__(256) void k_cell_scatter_fused(
     int n, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
     const int *__restrict__ cell_of, const int *__restrict__ cell_rank, int *__restrict__ cell_cnt,
@@ -809,7 +1025,8 @@ __global__ __launch_bounds__(256) void k_cell_scatter_fused(
         for (int c = t; c <= ncell; c += 256) cell_start[c] = lst[c];
     if (blockIdx.x >= nsc) {
         if (t <= 8) cut[t] = gcut;
-        order_runs(ncell, [&](int c) { return lst[c]; }, cell_work, order, hist, cut, blockIdx.x - nsc, 8);
+        order_runs(ncell, [&](int c) { return lst[c + 1] - lst[c]; }, cell_work, order, hist, cut, blockIdx.x - nsc,
+                   8);
     }
 
     if (p < n) {
@@ -1178,6 +1395,7 @@ struct CellsDev {
     int dense_on;  // 0: the dense-owner kernel does not follow; an owner that needs it fails the pass
     const int *order;  // each XCD's run of cells, heaviest first (k_cell_scatter)
     unsigned *cell_work;  // each cell's accepted triplets, the next sweep's dispatch key
+    SlotsDev sl;          // the slotted layout (cap > 0; c.cell_idx and xs, ys, zs then point at its slots)
 };
 
 // An owner that needs the dense-owner kernel when none follows this launch:
@@ -1191,7 +1409,9 @@ __device__ __forceinline__ void want_dense_kernel(Status *st)
 
 // The staged cells of cell c (one wave): c first, then its forward
 // neighbour cells.  Publishes each cell's (start, staged prefix) in cst/cpre,
-// cpre[nk] = total, and info = {own_start, own_cnt, nk, total}.
+// cpre[nk] = total, and info = {own_start, own_cnt, nk, total}.  With
+// scnt (the slotted layout, scap slots per cell) cell k starts at slot
+// k scap and holds scnt[k] particles.
 //
 // Which neighbour cells are "forward".  With >= 4 cells (or 1) in every
 // dimension, pairwise-adjacent cells lie in one unwrapped 2x2x2 block, so
@@ -1200,7 +1420,8 @@ __device__ __forceinline__ void want_dense_kernel(Status *st)
 // (the half shell) give each triplet exactly one owner.  With exactly 3 cells
 // in a dimension three cells can wrap into a cycle, so the cell index orders
 // them instead.
-__device__ void stage_cells(const CutDev &A, const SysGrid &sg, int c, int *cst, int *cpre, int *info)
+__device__ void stage_cells(const CutDev &A, const SysGrid &sg, int c, int *cst, int *cpre, int *info,
+                            const int *scnt = nullptr, int scap = 0)
 {
     const int lane = lane_id();
     const int ncx = sg.nc[0], ncy = sg.nc[1], ncz = sg.nc[2];
@@ -1221,12 +1442,21 @@ __device__ void stage_cells(const CutDev &A, const SysGrid &sg, int c, int *cst,
     const bool lexi = (ncx == 1 || ncx >= 4) && (ncy == 1 || ncy >= 4) && (ncz == 1 || ncz >= 4);
     const int centre = nst / 2;
     const bool fwd = cc >= 0 && (lexi ? lane > centre : cc > c);
-    const int own_start = A.cell_start[c];
-    const int own_cnt = A.cell_start[c + 1] - own_start;
-    int st = 0, n_in = 0;
-    if (fwd) {
-        st = A.cell_start[cc];
-        n_in = A.cell_start[cc + 1] - st;
+    int own_start, own_cnt, st = 0, n_in = 0;
+    if (scnt) {
+        own_start = c * scap;
+        own_cnt = min(scnt[c], scap);
+        if (fwd) {
+            st = cc * scap;
+            n_in = min(scnt[cc], scap);
+        }
+    } else {
+        own_start = A.cell_start[c];
+        own_cnt = A.cell_start[c + 1] - own_start;
+        if (fwd) {
+            st = A.cell_start[cc];
+            n_in = A.cell_start[cc + 1] - st;
+        }
     }
     const unsigned long long m = __ballot(fwd);
     int incl = n_in;
@@ -1317,6 +1547,20 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     // kernel.  (Earlier launches set both, so every thread reads the same
     // values.)
     if (A.st->halt) return;
+    // The slotted layout: this sweep's counts, and cell c's counter of the
+    // next binning cleared (every cell has one workgroup).  A cell that had
+    // more particles than slots fails the pass; the host re-runs the step
+    // on the contiguous layout.
+    const int *scnt = nullptr;
+    if (a.sl.cap > 0) {
+        const unsigned e = a.sl.state[0];
+        scnt = a.sl.cnt + (e & 1u) * a.ncell;
+        if (tid == 0) a.sl.cnt[((e + 1u) & 1u) * a.ncell + c] = 0;
+        if (A.st->pad_over) {
+            if (tid == 0) atomicOr(&A.st->flags[2], 1);  // the update kernel skips itself
+            return;
+        }
+    }
     if (A.st->flags[0] != 0) {
         if (tid == 0) {
             atomicOr(&A.st->flags[2], 1);  // the update kernel skips itself
@@ -1332,7 +1576,7 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
     unsigned cnt = 0;
 
     // 1. the staged cells: c first, then its forward neighbour cells
-    if (wv == 0) stage_cells(A, sg, c, cst, cpre, sinfo);
+    if (wv == 0) stage_cells(A, sg, c, cst, cpre, sinfo, scnt, a.sl.cap);
     __syncthreads();
     const int cand = sinfo[3], own = sinfo[1], nk = sinfo[2];
 
@@ -2535,6 +2779,22 @@ CutDev cut_dev(const DeviceState &d, const CellGrid &g, const CutoffSpec &c, dou
     return a;
 }
 
+// The slotted layout's device view (cap 0 when off).
+SlotsDev slots_dev(const CellGrid *g, int on)
+{
+    SlotsDev v{};
+    if (on && g && g->pad_cap > 0) {
+        v.cap = g->pad_cap;
+        v.cnt = g->pad_cnt;
+        v.state = g->pad_state;
+        v.idx = g->pad_idx;
+        v.x = g->pad_x;
+        v.y = g->pad_y;
+        v.z = g->pad_z;
+    }
+    return v;
+}
+
 unsigned owner_grid(int n) { return 8u * OWNER_STRETCH * (unsigned)((n + 8 * OWNER_STRETCH - 1) / (8 * OWNER_STRETCH)); }
 
 }  // namespace
@@ -2582,13 +2842,14 @@ void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h)
 }
 
 void launch_update_fin(hipStream_t s, const DeviceState &d, double hdt, const double g[3], Status *st, int sum_roles,
-                       int record, const CellGrid *cells)
+                       int record, const CellGrid *cells, int slotted)
 {
     // the particles' workgroups, then (with a cell list) one that cuts the next sweep's XCD runs
     const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + (cells ? 1 : 0);
     hipLaunchKernelGGL(k_update_fin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g[0], g[1], g[2], st, sum_roles, record,
                        cells ? cells->ncell : 0, cells ? cells->cell_start : nullptr,
-                       cells ? cells->cell_work : nullptr, cells ? cells->order : nullptr);
+                       cells ? cells->cell_work : nullptr, cells ? cells->order : nullptr,
+                       slots_dev(cells, slotted && cells));
 }
 
 void launch_sum_roles(hipStream_t s, const DeviceState &d)
@@ -2640,13 +2901,29 @@ void launch_scatter_fused(hipStream_t s, const DeviceState &d, CellGrid &g, Stat
 }
 
 void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const double g3[3], Status *st,
-                       int sum_roles, double dt, double h, int zero_f, CellGrid &g)
+                       int sum_roles, double dt, double h, int zero_f, CellGrid &g, int slotted)
 {
-    // the particles' workgroups, then one that cuts the next sweep's XCD runs
-    const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + 1;
+    // the particles' workgroups, then one that cuts the next sweep's XCD
+    // runs (slotted: 8 that cut them and order one run each)
+    const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + (slotted ? 8 : 1);
     hipLaunchKernelGGL(k_update_bin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g3[0], g3[1], g3[2], st, sum_roles, dt,
                        h, zero_f, grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank, g.ncell, g.cell_start,
-                       g.cell_work, g.order);
+                       g.cell_work, g.order, slots_dev(&g, slotted));
+}
+
+void launch_slots_begin(hipStream_t s, CellGrid &g, Status *st)
+{
+    hipLaunchKernelGGL(k_slots_begin, dim3(blocks_for(2LL * g.ncell, 256)), dim3(256), 0, s, g.ncell,
+                       slots_dev(&g, 1), st);
+}
+
+void launch_bin_slotted(hipStream_t s, const DeviceState &d, double dt, double h, int zero_f, CellGrid &g,
+                        Status *st)
+{
+    // the particles' workgroups, then 8 that order the first sweep's cells
+    const unsigned nb = (d.n > 0 ? blocks_for(d.n, 256) : 0) + 8;
+    hipLaunchKernelGGL(k_bin_slotted, dim3(nb), dim3(256), 0, s, d, dt, h, zero_f, grid_dev(g), slots_dev(&g, 1),
+                       st, g.ncell, g.cell_work, g.order);
 }
 
 void launch_cells_build(hipStream_t s, const DeviceState &d, int predict, double dt, double h, int zero_f,
@@ -2710,7 +2987,8 @@ void launch_p3bca_fast(hipStream_t s, const DeviceState &d, const CellGrid &g, c
 }
 
 void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, const CutoffSpec &c, double nu,
-                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int dense_on)
+                        int mixed, int grad, Status *st, int count_slot, int want_energy, int *owners, int dense_on,
+                        int slotted)
 {
     if (d.n <= 0) return;
     CellsDev a;
@@ -2724,6 +3002,13 @@ void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     a.owners = owners;
     a.order = g.order;
     a.cell_work = g.cell_work;
+    a.sl = slots_dev(&g, slotted);
+    if (slotted) {
+        a.c.cell_idx = g.pad_idx;
+        a.xs = g.pad_x;
+        a.ys = g.pad_y;
+        a.zs = g.pad_z;
+    }
     // the 8 XCDs' runs at their capacity (the workgroups past a run's length exit at once)
     const dim3 gr((unsigned)(8 * cell_run_cap(g.ncell))), blk(CELL_WAVES * WAVE);
 #define TB3_CELLS(T, G, E) hipLaunchKernelGGL((k_p3bca_cells<T, G, E>), gr, blk, 0, s, a)

@@ -238,6 +238,18 @@ class Engine:
         """Fix the knobs (cell_ratio in [1, 4]; dense_waves 4, 8 or 16)."""
         self._check(self._lib.tb3_set_tuning(self._ctx, float(cell_ratio), int(dense_waves)))
 
+    def set_cell_slots(self, slots=-1):
+        """Slots per cell of the slotted cell layout that replayed steps
+        (run) use on the fast cutoff path: -1 automatic, 0 off, 1 .. 256."""
+        self._check(self._lib.tb3_set_cell_slots(self._ctx, int(slots)))
+
+    def get_cell_slots(self):
+        """(requested, in_use): the setting, and the slots per cell of the
+        last captured step graph (0: contiguous layout)."""
+        req, use = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self._lib.tb3_get_cell_slots(self._ctx, ctypes.byref(req), ctypes.byref(use)))
+        return req.value, use.value
+
     # -- phase timing ---------------------------------------------------------
     def set_profiling(self, enable=True):
         self._check(self._lib.tb3_set_profiling(self._ctx, 1 if enable else 0))
