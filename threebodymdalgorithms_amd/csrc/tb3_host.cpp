@@ -772,12 +772,15 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
     DeviceState &d = c->state.d;
     const int per = c->nrep > 1 ? c->nrep : 1;
     if ((long long)steps * per > c->counts_cap) {
-        dev_alloc(c->d_counts, (long long)steps * per);
+        // room for 4096 counts at least, and twice what is asked: a run
+        // longer than the last one must not pay for a device and a pinned
+        // host allocation (~0.15 ms together) on its own clock
+        const long long cap = std::max<long long>(2LL * steps * per, 4096);
+        dev_alloc(c->d_counts, cap);
         if (c->h_counts) (void)hipHostFree(c->h_counts);
         c->h_counts = nullptr;
-        HIPCHK(hipHostMalloc((void **)&c->h_counts, (size_t)steps * per * sizeof(unsigned long long),
-                             hipHostMallocDefault));
-        c->counts_cap = (long long)steps * per;
+        HIPCHK(hipHostMalloc((void **)&c->h_counts, (size_t)cap * sizeof(unsigned long long), hipHostMallocDefault));
+        c->counts_cap = cap;
     }
     int done = 0;
     while (done < steps) {
