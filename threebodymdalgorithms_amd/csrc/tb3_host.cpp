@@ -185,8 +185,10 @@ struct tb3_ctx {
     // cutoff grids that scan inside the scatter kernel replay the steps as
     // one launch of the binning, then step_exec_mid (scatter, sweep, the
     // update fused with the next step's binning) for all steps but the
-    // last, and step_exec (scatter, sweep, update) for the last
+    // last, GRAPH_BATCH of them per launch of step_exec_batch while that
+    // many remain, and step_exec (scatter, sweep, update) for the last
     hipGraphExec_t step_exec_mid = nullptr;
+    hipGraphExec_t step_exec_batch = nullptr;
     bool graph_fused = false;
     unsigned long long epoch = 1;  // bumped whenever buffers, grid or parameters change
     unsigned long long graph_epoch = 0;
@@ -656,8 +658,12 @@ unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool inte
     return c->h_status->count[0];
 }
 
+// Steps per launch of the batched middle-step graph.
+constexpr int GRAPH_BATCH = 8;
+
 // Replays of one captured step graph run `steps` steps of tb3_run with one
-// launch per step and one host synchronisation per run.  A step whose sweep
+// launch per step (per GRAPH_BATCH steps where the batched graph applies)
+// and one host synchronisation per run.  A step whose sweep
 // overflows halts the replays that follow (they leave state and flags
 // alone); the host completes that step with the overflow protocol and
 // captures again for the rest.
@@ -719,6 +725,10 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
         (void)hipGraphExecDestroy(c->step_exec_mid);
         c->step_exec_mid = nullptr;
     }
+    if (c->step_exec_batch) {
+        (void)hipGraphExecDestroy(c->step_exec_batch);
+        c->step_exec_batch = nullptr;
+    }
 
     hipStream_t s = c->stream;
     DeviceState &d = c->state.d;
@@ -732,6 +742,11 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
                               c->grid, slotted ? 1 : 0);
         };
         c->step_exec_mid = capture(s, mid);
+        // about 8 us pass between one graph launch's last kernel and the
+        // next one's first, none between the kernels of one graph
+        c->step_exec_batch = capture(s, [&] {
+            for (int k = 0; k < GRAPH_BATCH; ++k) mid();
+        });
         c->step_exec = capture(s, [&] {
             if (!slotted) launch_scatter_fused(s, d, c->grid, c->d_status);
             launch_sweep(c, s, d, true, false, false, slotted);
@@ -809,7 +824,9 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
                 launch_bin_ranked(s, d, 1, dt, dt * dt * 0.5, c->precision == TB3_FP64_EXACT ? 0 : 1, c->grid,
                                   c->d_status);
             }
-            for (int k = done; k + 1 < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec_mid, s));
+            int k = done;
+            for (; k + GRAPH_BATCH < steps; k += GRAPH_BATCH) HIPCHK(hipGraphLaunch(c->step_exec_batch, s));
+            for (; k + 1 < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec_mid, s));
             HIPCHK(hipGraphLaunch(c->step_exec, s));
         } else {
             for (int k = done; k < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec, s));
@@ -995,6 +1012,7 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.rep_energy);
     if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
     if (c->step_exec_mid) (void)hipGraphExecDestroy(c->step_exec_mid);
+    if (c->step_exec_batch) (void)hipGraphExecDestroy(c->step_exec_batch);
     for (hipEvent_t &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_status) (void)hipHostFree(c->h_status);
