@@ -445,10 +445,7 @@ void ensure_slot_buffers(tb3_ctx *c, int n)
     const long long need = (long long)g.ncell * g.pad_cap;
     if (need > c->slot_buf_cap) {
         ++c->epoch;
-        dev_alloc(g.pad_idx, need);
-        dev_alloc(g.pad_x, need);
-        dev_alloc(g.pad_y, need);
-        dev_alloc(g.pad_z, need);
+        dev_alloc(g.pad_rec, need);
         c->slot_buf_cap = need;
     }
     if (2 * g.ncell > c->slot_cnt_cap) {
@@ -993,10 +990,7 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.zs);
     dev_free(c->grid.pad_cnt);
     dev_free(c->grid.pad_state);
-    dev_free(c->grid.pad_idx);
-    dev_free(c->grid.pad_x);
-    dev_free(c->grid.pad_y);
-    dev_free(c->grid.pad_z);
+    dev_free(c->grid.pad_rec);
     dev_free(c->scan_tmp);
     dev_free(c->triples);
     dev_free(c->d_status);

@@ -71,8 +71,7 @@ struct CellGrid {
     int pad_cap = 0;                 // slots per cell; 0: no slotted buffers
     int *pad_cnt = nullptr;          // 2 ncell
     unsigned *pad_state = nullptr;   // [0] epoch, [1] finished workgroups of a binning pass (returns to 0)
-    int *pad_idx = nullptr;          // ncell pad_cap particle indices
-    double *pad_x = nullptr, *pad_y = nullptr, *pad_z = nullptr;  // ncell pad_cap positions
+    double4 *pad_rec = nullptr;      // ncell pad_cap slots: x, y, z and the particle index (its bits in w)
 };
 
 struct CutoffSpec {

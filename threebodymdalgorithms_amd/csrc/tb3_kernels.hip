@@ -445,8 +445,7 @@ struct SlotsDev {
     int cap;          // slots per cell; 0: the contiguous layout is in use
     int *cnt;         // 2 ncell counts, by epoch parity
     unsigned *state;  // [0] epoch, [1] finished workgroups of a binning pass
-    int *idx;         // ncell cap particle indices
-    double *x, *y, *z;
+    double4 *rec;     // ncell cap slots: x, y, z, and the particle index's bits in w
 };
 
 // Particle p (position px, py, pz) into a slot of cell c, counted in cnt; a
@@ -456,11 +455,8 @@ __device__ __forceinline__ void bin_slotted(const SlotsDev &sl, int *cnt, int c,
 {
     const int r = atomicAdd(&cnt[c], 1);
     if (r < sl.cap) {
-        const long long k = (long long)c * sl.cap + r;
-        sl.idx[k] = p;
-        sl.x[k] = px;
-        sl.y[k] = py;
-        sl.z[k] = pz;
+        // one 32-byte record: a single store where four arrays took four
+        sl.rec[(long long)c * sl.cap + r] = make_double4(px, py, pz, __longlong_as_double((long long)p));
     } else {
         atomicOr(&st->pad_over, 1);
     }
@@ -1395,7 +1391,7 @@ struct CellsDev {
     int dense_on;  // 0: the dense-owner kernel does not follow; an owner that needs it fails the pass
     const int *order;  // each XCD's run of cells, heaviest first (k_cell_scatter)
     unsigned *cell_work;  // each cell's accepted triplets, the next sweep's dispatch key
-    SlotsDev sl;          // the slotted layout (cap > 0; c.cell_idx and xs, ys, zs then point at its slots)
+    SlotsDev sl;          // the slotted layout (cap > 0: staged from its slot records)
 };
 
 // An owner that needs the dense-owner kernel when none follows this launch:
@@ -1626,10 +1622,18 @@ __global__ __launch_bounds__(CELL_WAVES *WAVE, 4) void k_p3bca_cells(CellsDev a)
                 q[h] = 0;
                 if (t < cand) {
                     const int slot = staged_slot(cst, cpre, nk, t);
-                    x[h] = a.xs[slot];
-                    y[h] = a.ys[slot];
-                    z[h] = a.zs[slot];
-                    q[h] = A.cell_idx[slot];
+                    if (scnt) {
+                        const double4 rc = a.sl.rec[slot];
+                        x[h] = rc.x;
+                        y[h] = rc.y;
+                        z[h] = rc.z;
+                        q[h] = (int)__double_as_longlong(rc.w);
+                    } else {
+                        x[h] = a.xs[slot];
+                        y[h] = a.ys[slot];
+                        z[h] = a.zs[slot];
+                        q[h] = A.cell_idx[slot];
+                    }
                 }
             }
 #pragma unroll
@@ -2787,10 +2791,7 @@ SlotsDev slots_dev(const CellGrid *g, int on)
         v.cap = g->pad_cap;
         v.cnt = g->pad_cnt;
         v.state = g->pad_state;
-        v.idx = g->pad_idx;
-        v.x = g->pad_x;
-        v.y = g->pad_y;
-        v.z = g->pad_z;
+        v.rec = g->pad_rec;
     }
     return v;
 }
@@ -3003,12 +3004,6 @@ void launch_p3bca_cells(hipStream_t s, const DeviceState &d, const CellGrid &g, 
     a.order = g.order;
     a.cell_work = g.cell_work;
     a.sl = slots_dev(&g, slotted);
-    if (slotted) {
-        a.c.cell_idx = g.pad_idx;
-        a.xs = g.pad_x;
-        a.ys = g.pad_y;
-        a.zs = g.pad_z;
-    }
     // the 8 XCDs' runs at their capacity (the workgroups past a run's length exit at once)
     const dim3 gr((unsigned)(8 * cell_run_cap(g.ncell))), blk(CELL_WAVES * WAVE);
 #define TB3_CELLS(T, G, E) hipLaunchKernelGGL((k_p3bca_cells<T, G, E>), gr, blk, 0, s, a)
