@@ -423,12 +423,14 @@ void ensure_grid_buffers(tb3_ctx *c, int n)
 
 // Slots per cell of the slotted layout for n particles: three times the
 // mean cell count (at least 32, a multiple of 16) unless set by
-// tb3_set_cell_slots; 0 where the layout does not apply (off, failed, a
-// replica batch, or cells too full to stage).
+// tb3_set_cell_slots; 0 where the layout does not apply (off, failed, the
+// exact path, a replica batch, or cells too full to stage).
 int slots_for(const tb3_ctx *c, int n)
 {
     const CellGrid &g = c->grid;
-    if (c->slots_req == 0 || c->slots_failed || g.nrep > 1 || g.ncell <= 0 || n < 3) return 0;
+    if (c->slots_req == 0 || c->slots_failed || c->precision == TB3_FP64_EXACT || g.nrep > 1 || g.ncell <= 0 ||
+        n < 3)
+        return 0;
     if (c->slots_req > 0) return c->slots_req;
     const int mean = (int)((n + (long long)g.ncell - 1) / g.ncell);
     const int cap = std::max(32, (3 * mean + 15) / 16 * 16);
