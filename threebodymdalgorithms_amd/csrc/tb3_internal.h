@@ -25,13 +25,14 @@ struct DeviceState {
 // runs cut to about equal work (so their lengths vary around ncell / 8),
 // each sorted heaviest first.  `order` holds 8 runs of cell_run_cap(ncell)
 // entries, then the 8 run lengths, then the 9 cuts (run r is cells
-// [cut r, cut r+1)) that the last replayed step's update computed.
+// [cut r, cut r+1)) that the last integrating pass's update computed, then
+// two more sets of 9 that replays on the slotted layout alternate between.
 __host__ __device__ inline int cell_run_cap(int ncell)
 {
     const int run = (ncell + 7) / 8;
     return run + (run + 1) / 2;
 }
-__host__ __device__ inline long long cell_order_len(int ncell) { return 8LL * cell_run_cap(ncell) + 17; }
+__host__ __device__ inline long long cell_order_len(int ncell) { return 8LL * cell_run_cap(ncell) + 35; }
 
 struct CellGrid {
     int nc[3] = {1, 1, 1};

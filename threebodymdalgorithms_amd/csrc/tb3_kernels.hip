@@ -361,85 +361,6 @@ __device__ void order_runs(int ncell, const Count &count, const unsigned *__rest
 // kernel (k_cell_scatter_fused) and may replay on the slotted layout.
 constexpr int FUSED_SCAN_CELLS = 8192;
 
-// compute_cuts and order_runs in one workgroup of 256 threads for run r,
-// on per-cell weights loaded once into LDS (w: ncell floats): the same cuts
-// (written to out when write_out) and the same order, with a single round
-// trip to memory and 256-bin scans done in the waves.  LDS besides w:
-// fpart (4 floats), cut (9 ints), hist (256 ints), ipart (4 ints).
-template <class Count>
-__device__ void cut_and_order(int ncell, const Count &count, const unsigned *__restrict__ cell_work, int *out,
-                              bool write_out, int *__restrict__ order, int r, float *w, float *fpart, int *cut,
-                              int *hist, int *ipart)
-{
-    const int t = threadIdx.x, lane = t & (WAVE - 1), wv = t >> 6;
-    const int run = (ncell + 7) >> 3, cap = cell_run_cap(ncell);
-    for (int c = t; c < ncell; c += 256) {
-        const unsigned k = cell_work[c];
-        const float n = (float)count(c);
-        w[c] = k ? (float)k : 16.0f * n * n;
-    }
-    if (t <= 8) cut[t] = ncell;
-    __syncthreads();
-    // the cuts: chunk sums, their scan, each cell's run (as compute_cuts)
-    const int per = (ncell + 255) >> 8;
-    const int c0 = min(t * per, ncell), c1 = min(c0 + per, ncell);
-    float s = 0.0f;
-    for (int c = c0; c < c1; ++c) s += w[c];
-    float incl = s;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-        const float u = __shfl_up(incl, o);
-        if (lane >= o) incl += u;
-    }
-    if (lane == WAVE - 1) fpart[wv] = incl;
-    __syncthreads();
-    float before = 0.0f;
-    for (int k = 0; k < wv; ++k) before += fpart[k];
-    const float total = (fpart[0] + fpart[1]) + (fpart[2] + fpart[3]);
-    if (total > 0.0f) {
-        float acc = before + incl - s;
-        for (int c = c0; c < c1; ++c) {
-            const float wc = w[c];
-            const int rr = min(7, max(0, (int)(8.0f * (acc + 0.5f * wc) / total)));
-            atomicMin(&cut[rr], c);
-            acc += wc;
-        }
-    }
-    __syncthreads();
-    if (t == 0) {
-        bool ok = total > 0.0f;
-        cut[0] = 0;
-        for (int k = 7; k >= 1; --k) cut[k] = min(cut[k], cut[k + 1]);
-        for (int k = 0; k < 8 && ok; ++k) ok = cut[k + 1] - cut[k] <= cap;
-        for (int k = 0; k <= 8; ++k) {
-            cut[k] = ok ? cut[k] : min(k * run, ncell);
-            if (write_out) out[k] = cut[k];
-        }
-    }
-    __syncthreads();
-    // run r heaviest first: a counting sort on bins of 1/16 octave of the work
-    auto bin = [&](int c) { return 255 - min(255, (int)(16.0f * __log2f(1.0f + w[c]))); };
-    const int lo = cut[r], hi = cut[r + 1];
-    hist[t] = 0;
-    __syncthreads();
-    for (int c = lo + t; c < hi; c += 256) atomicAdd(&hist[bin(c)], 1);
-    __syncthreads();
-    const int own = hist[t];
-    int v = own;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-        const int u = __shfl_up(v, o);
-        if (lane >= o) v += u;
-    }
-    if (lane == WAVE - 1) ipart[wv] = v;
-    __syncthreads();
-    for (int k = 0; k < wv; ++k) v += ipart[k];
-    hist[t] = v - own;
-    __syncthreads();
-    for (int c = lo + t; c < hi; c += 256) order[r * cap + atomicAdd(&hist[bin(c)], 1)] = c;
-    if (t == 0) order[8 * cap + r] = hi - lo;
-}
-
 // The slotted cell layout (CellGrid::pad_cap) as the kernels see it.
 struct SlotsDev {
     int cap;          // slots per cell; 0: the contiguous layout is in use
@@ -780,8 +701,11 @@ __global__ __launch_bounds__(256) void k_bin_slotted(DeviceState d, double dt, d
     const int nbin = (int)gridDim.x - 8;
     if ((int)blockIdx.x >= nbin) {
         __shared__ int hist[256], cut[9];
-        if (threadIdx.x <= 8) cut[threadIdx.x] = order[8 * cell_run_cap(ncell) + 8 + threadIdx.x];
+        int *cuts = order + 8 * cell_run_cap(ncell) + 8;
+        if (threadIdx.x <= 8) cut[threadIdx.x] = cuts[threadIdx.x];
         order_runs(ncell, [](int) { return 0; }, cell_work, order, hist, cut, (int)blockIdx.x - nbin, 8);
+        // the cuts the first replayed update orders with (epoch 0's set)
+        if (blockIdx.x == nbin && threadIdx.x <= 8) cuts[9 + threadIdx.x] = cut[threadIdx.x];
         return;
     }
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -880,36 +804,23 @@ __global__ __launch_bounds__(NSLOT) void k_update_bin(DeviceState d, double hdt,
                                                      const unsigned *__restrict__ cell_work, int *__restrict__ order,
                                                      SlotsDev sl)
 {
+    // The particles' workgroups, then: (slotted) 8 that order the next
+    // sweep's runs; one that cuts the XCD runs from this sweep's work; one
+    // that folds the status.  flags[2] is the same for the whole grid: only
+    // a halted replay has it, and then nothing but the fold runs and no
+    // workgroup takes a ticket (the fold's status clear happens only when
+    // flags[2] is already zero).
     const bool slotted = sl.cap > 0;
-    const int nupd = (int)gridDim.x - (slotted ? 8 : 1);  // the particles' workgroups
-    if ((int)blockIdx.x >= nupd) {
-        __shared__ float fpart[4], wl[FUSED_SCAN_CELLS];
-        __shared__ int cut[9], hist[256], ipart[4];
-        int *out = order + 8 * cell_run_cap(ncell) + 8;
-        if (!slotted) {
-            // one workgroup cuts the next sweep's XCD runs from this sweep's
-            // per-cell work (the scatter kernel then orders them)
-            compute_cuts(ncell, [&](int c) { return cell_start[c + 1] - cell_start[c]; }, cell_work, out, fpart,
-                         cut);
-            return;
-        }
-        // slotted (no scatter kernel follows): 8 workgroups each cut the runs
-        // and order one of them for the next sweep, from this sweep's work
-        // and counts.  flags[2] is the same for the whole grid (only a
-        // halted replay has it, and then nothing here runs or takes a ticket).
-        if (st->flags[2]) return;
-        const unsigned e = sl.state[0];
-        const int *cur = sl.cnt + (e & 1u) * ncell;
-        auto count = [&](int c) { return min(cur[c], sl.cap); };
-        const int r = (int)blockIdx.x - nupd;
-        cut_and_order(ncell, count, cell_work, out, r == 0, order, r, wl, fpart, cut, hist, ipart);
-        slots_ticket(sl, e);
-        return;
-    }
-    if (blockIdx.x == 0) {
+    const int nupd = (int)gridDim.x - (slotted ? 10 : 2);
+    const int x = (int)blockIdx.x - nupd;  // >= 0: an extra workgroup
+    const unsigned e = slotted ? sl.state[0] : 0u;
+    if (x == (slotted ? 9 : 1)) {
+        // the status fold, the step's count record and the clear for the
+        // next replay, beside the particles' workgroups
         fold_slots(st);
         __syncthreads();  // thread 0's fold (and halt) is visible to the workgroup
-        if (!st->halt) {
+        const bool halted = st->halt;
+        if (!halted) {
             const int t = threadIdx.x, nrep = st->nrep, ns = st->nstep;
             if (nrep > 1) {
                 for (int r = t; r < nrep; r += NSLOT) st->counts[(long long)ns * nrep + r] = st->rep_count[r];
@@ -920,9 +831,38 @@ __global__ __launch_bounds__(NSLOT) void k_update_bin(DeviceState d, double hdt,
             clear_status(st, true, t, NSLOT);
             if (t == 0) st->nstep = ns + 1;
         }
+        if (slotted && !halted) slots_ticket(sl, e);
+        return;
+    }
+    if (x >= 0) {
+        __shared__ float fpart[4];
+        __shared__ int cut[9], hist[256];
+        int *cuts = order + 8 * cell_run_cap(ncell) + 8;
+        if (!slotted) {
+            // the next sweep's XCD runs from this sweep's per-cell work (the
+            // scatter kernel then orders them)
+            compute_cuts(ncell, [&](int c) { return cell_start[c + 1] - cell_start[c]; }, cell_work, cuts, fpart,
+                         cut);
+            return;
+        }
+        if (st->flags[2]) return;
+        const int *cur = sl.cnt + (e & 1u) * ncell;
+        auto count = [&](int c) { return min(cur[c], sl.cap); };
+        // slotted runs keep two more sets of cuts, by epoch parity: the
+        // orders below use the cuts of the previous replay's work (the set of
+        // this epoch), while the cut workgroup computes the next set from this
+        // sweep's, so neither waits for the other
+        int *par = cuts + 9;
+        if (x < 8) {
+            if (threadIdx.x <= 8) cut[threadIdx.x] = par[(e & 1u) * 9 + threadIdx.x];
+            order_runs(ncell, count, cell_work, order, hist, cut, x, 8);
+        } else {
+            compute_cuts(ncell, count, cell_work, par + ((e + 1u) & 1u) * 9, fpart, cut);
+        }
+        slots_ticket(sl, e);
+        return;
     }
     if (st->flags[2]) return;
-    const unsigned e = slotted ? sl.state[0] : 0u;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < d.n) update_bin_one(d, hdt, gx, gy, gz, st, sum_roles, dt, h, zero_f, g, cell_of, cell_cnt, cell_rank,
                                 sl, slotted ? sl.cnt + ((e + 1u) & 1u) * ncell : nullptr, p);
@@ -2904,9 +2844,9 @@ void launch_scatter_fused(hipStream_t s, const DeviceState &d, CellGrid &g, Stat
 void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const double g3[3], Status *st,
                        int sum_roles, double dt, double h, int zero_f, CellGrid &g, int slotted)
 {
-    // the particles' workgroups, then one that cuts the next sweep's XCD
-    // runs (slotted: 8 that cut them and order one run each)
-    const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + (slotted ? 8 : 1);
+    // the particles' workgroups, then (slotted) 8 that order the next
+    // sweep's runs, one that cuts the runs and one that folds the status
+    const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + (slotted ? 10 : 2);
     hipLaunchKernelGGL(k_update_bin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g3[0], g3[1], g3[2], st, sum_roles, dt,
                        h, zero_f, grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank, g.ncell, g.cell_start,
                        g.cell_work, g.order, slots_dev(&g, slotted));
