@@ -411,23 +411,12 @@ __global__ __launch_bounds__(NSLOT) void k_update_fin(DeviceState d, double hdt,
                                                      const unsigned *__restrict__ cell_work, int *__restrict__ order,
                                                      SlotsDev sl)
 {
-    // with a cell list (ncell > 0) the last workgroup cuts the next sweep's
-    // XCD runs from this sweep's per-cell work (and cell counts, contiguous
-    // or slotted)
-    if (ncell > 0 && blockIdx.x == gridDim.x - 1) {
-        __shared__ float fpart[4];
-        __shared__ int cut[9];
-        int *out = order + 8 * cell_run_cap(ncell) + 8;
-        if (sl.cap > 0) {
-            const int *cur = sl.cnt + (sl.state[0] & 1u) * ncell;
-            compute_cuts(ncell, [&](int c) { return min(cur[c], sl.cap); }, cell_work, out, fpart, cut);
-        } else {
-            compute_cuts(ncell, [&](int c) { return cell_start[c + 1] - cell_start[c]; }, cell_work, out, fpart,
-                         cut);
-        }
-        return;
-    }
-    if (blockIdx.x == 0) {
+    // The particles' workgroups, then (with a cell list, ncell > 0) one that
+    // cuts the next sweep's XCD runs from this sweep's per-cell work (and
+    // cell counts, contiguous or slotted), then one that folds the status.
+    const int nupd = (int)gridDim.x - (ncell > 0 ? 2 : 1);
+    const int x = (int)blockIdx.x - nupd;
+    if (x == (ncell > 0 ? 1 : 0)) {
         fold_slots(st);
         __syncthreads();  // thread 0's fold (and halt) is visible to the workgroup
         if (record && !st->halt) {
@@ -441,7 +430,23 @@ __global__ __launch_bounds__(NSLOT) void k_update_fin(DeviceState d, double hdt,
             clear_status(st, false, t, NSLOT);
             if (t == 0) st->nstep = ns + 1;
         }
+        return;
     }
+    if (x == 0) {
+        __shared__ float fpart[4];
+        __shared__ int cut[9];
+        int *out = order + 8 * cell_run_cap(ncell) + 8;
+        if (sl.cap > 0) {
+            const int *cur = sl.cnt + (sl.state[0] & 1u) * ncell;
+            compute_cuts(ncell, [&](int c) { return min(cur[c], sl.cap); }, cell_work, out, fpart, cut);
+        } else {
+            compute_cuts(ncell, [&](int c) { return cell_start[c + 1] - cell_start[c]; }, cell_work, out, fpart,
+                         cut);
+        }
+        return;
+    }
+    // The fold's clear happens only when flags[2] is already zero, so every
+    // workgroup reads the same flags[2].
     if (st->flags[2]) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.n) return;
@@ -2785,8 +2790,9 @@ void launch_predict(hipStream_t s, const DeviceState &d, double dt, double h)
 void launch_update_fin(hipStream_t s, const DeviceState &d, double hdt, const double g[3], Status *st, int sum_roles,
                        int record, const CellGrid *cells, int slotted)
 {
-    // the particles' workgroups, then (with a cell list) one that cuts the next sweep's XCD runs
-    const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + (cells ? 1 : 0);
+    // the particles' workgroups, then (with a cell list) one that cuts the
+    // next sweep's XCD runs, and one that folds the status
+    const unsigned nb = (d.n > 0 ? blocks_for(d.n, NSLOT) : 1) + (cells ? 1 : 0) + 1;
     hipLaunchKernelGGL(k_update_fin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g[0], g[1], g[2], st, sum_roles, record,
                        cells ? cells->ncell : 0, cells ? cells->cell_start : nullptr,
                        cells ? cells->cell_work : nullptr, cells ? cells->order : nullptr,
