@@ -360,3 +360,39 @@ def test_empty_and_tiny_systems(eng):
         assert nt == 0
         st = eng.get_particles()
         assert np.array_equal(st["f"], np.zeros((n, 3)))
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp64"])
+def test_nonfinite_position(eng, oracle, precision):
+    """A NaN coordinate fails every cutoff test's "greater than", so the
+    reference pairs that particle with everything (Algorithm.cpp:95-97): its
+    triplets with each pair within the cutoff count, and every force they
+    reach is NaN.  The cell kernel hands such a sweep to the per-owner kernel
+    over all particles; counts equal the oracle's brute sweep, NaN rows
+    coincide, finite rows meet the path's bar.  A replayed run (slotted cell
+    layout on the fast path) that meets the NaN at its first binning counts
+    the same per step as steps taken one at a time."""
+    P = uniform_particles(300, 8.0, 77)
+    P[17, 1] = np.nan
+    L = np.array([8.0, 8.0, 8.0])
+    say(f"non-finite position {precision}")
+    f, cnt = gpu_forces(eng, P, "p3bca", 2.0, precision, L=L)
+    *_, fo, co = oracle.sweep(P, cutoff=2.0, L=L, mode="brute")
+    assert cnt == co
+    bad = np.isnan(fo).any(axis=1)
+    assert bad.any()  # here every particle shares a triplet with the NaN one
+    assert np.array_equal(np.isnan(f).any(axis=1), bad)
+    if (~bad).any():
+        if precision == "exact":
+            assert np.array_equal(f[~bad], fo[~bad])
+        else:
+            assert force_errors(f[~bad], fo[~bad])["max_rel_floored"] <= 1e-12
+    V = np.zeros_like(P)
+    G = np.array([0.0, -9.81, 0.0])
+    eng.set_particles(P, V, V, np.ones(len(P)))
+    eng.configure("p3bca", cutoff=2.0, domain_L=L, precision=precision)
+    c_steps = [eng.step(0.001, G)[1] for _ in range(3)]
+    eng.set_particles(P, V, V, np.ones(len(P)))
+    c_run = [int(c) for c in eng.run(3, 0.001, G)]
+    say(f"  counts steps {c_steps} run {c_run}")
+    assert c_run == c_steps
