@@ -44,7 +44,7 @@ struct CellGrid {
     int *cell_rank = nullptr;   // n: each particle's rank in its cell (grids of at most 8192 cells)
     unsigned *scatter_ticket = nullptr;  // 1: finished scatter workgroups (returns to 0)
     int *cell_start = nullptr;  // ncell + 1
-    int *order = nullptr;       // ncell: each XCD's run of (ncell + 7) / 8 cells, heaviest first
+    int *order = nullptr;       // cell_order_len(ncell): the runs, their lengths and the cuts (above)
     unsigned *cell_work = nullptr;  // ncell: accepted triplets of each cell's last sweep (0: none yet)
     int *cell_idx = nullptr;    // n, particle indices grouped by cell (order within a cell unspecified)
     double *xs = nullptr, *ys = nullptr, *zs = nullptr;  // n, positions in cell_idx order
