@@ -185,10 +185,10 @@ struct tb3_ctx {
     // cutoff grids that scan inside the scatter kernel replay the steps as
     // one launch of the binning, then step_exec_mid (scatter, sweep, the
     // update fused with the next step's binning) for all steps but the
-    // last, GRAPH_BATCH of them per launch of step_exec_batch while that
-    // many remain, and step_exec (scatter, sweep, update) for the last
+    // last, 2^k of them per launch of step_exec_batch[k] (k = 1 .. 4,
+    // largest first), and step_exec (scatter, sweep, update) for the last
     hipGraphExec_t step_exec_mid = nullptr;
-    hipGraphExec_t step_exec_batch = nullptr;
+    hipGraphExec_t step_exec_batch[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     bool graph_fused = false;
     unsigned long long epoch = 1;  // bumped whenever buffers, grid or parameters change
     unsigned long long graph_epoch = 0;
@@ -657,11 +657,11 @@ unsigned long long run_pass(tb3_ctx *c, hipStream_t s, DeviceState &d, bool inte
     return c->h_status->count[0];
 }
 
-// Steps per launch of the batched middle-step graph.
-constexpr int GRAPH_BATCH = 8;
+// The batched middle-step graphs hold 2, 4, 8 and 16 steps.
+constexpr int GRAPH_BATCH_LOG2 = 4;
 
 // Replays of one captured step graph run `steps` steps of tb3_run with one
-// launch per step (per GRAPH_BATCH steps where the batched graph applies)
+// launch per step (per up to 16 steps where the batched graphs apply)
 // and one host synchronisation per run.  A step whose sweep
 // overflows halts the replays that follow (they leave state and flags
 // alone); the host completes that step with the overflow protocol and
@@ -724,9 +724,9 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
         (void)hipGraphExecDestroy(c->step_exec_mid);
         c->step_exec_mid = nullptr;
     }
-    if (c->step_exec_batch) {
-        (void)hipGraphExecDestroy(c->step_exec_batch);
-        c->step_exec_batch = nullptr;
+    for (hipGraphExec_t &b : c->step_exec_batch) {
+        if (b) (void)hipGraphExecDestroy(b);
+        b = nullptr;
     }
 
     hipStream_t s = c->stream;
@@ -743,9 +743,10 @@ void capture_step_graph(tb3_ctx *c, double dt, const double *g)
         c->step_exec_mid = capture(s, mid);
         // about 8 us pass between one graph launch's last kernel and the
         // next one's first, none between the kernels of one graph
-        c->step_exec_batch = capture(s, [&] {
-            for (int k = 0; k < GRAPH_BATCH; ++k) mid();
-        });
+        for (int k = 1; k <= GRAPH_BATCH_LOG2; ++k)
+            c->step_exec_batch[k] = capture(s, [&] {
+                for (int j = 0; j < (1 << k); ++j) mid();
+            });
         c->step_exec = capture(s, [&] {
             if (!slotted) launch_scatter_fused(s, d, c->grid, c->d_status);
             launch_sweep(c, s, d, true, false, false, slotted);
@@ -824,7 +825,9 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
                                   c->d_status);
             }
             int k = done;
-            for (; k + GRAPH_BATCH < steps; k += GRAPH_BATCH) HIPCHK(hipGraphLaunch(c->step_exec_batch, s));
+            // the middle steps in batches of 16, 8, 4, 2, then one
+            for (int b = GRAPH_BATCH_LOG2; b >= 1; --b)
+                for (; k + (1 << b) < steps; k += 1 << b) HIPCHK(hipGraphLaunch(c->step_exec_batch[b], s));
             for (; k + 1 < steps; ++k) HIPCHK(hipGraphLaunch(c->step_exec_mid, s));
             HIPCHK(hipGraphLaunch(c->step_exec, s));
         } else {
@@ -1008,7 +1011,8 @@ tb3_status tb3_destroy(tb3_ctx *c)
     dev_free(c->grid.rep_energy);
     if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
     if (c->step_exec_mid) (void)hipGraphExecDestroy(c->step_exec_mid);
-    if (c->step_exec_batch) (void)hipGraphExecDestroy(c->step_exec_batch);
+    for (hipGraphExec_t b : c->step_exec_batch)
+        if (b) (void)hipGraphExecDestroy(b);
     for (hipEvent_t &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_status) (void)hipHostFree(c->h_status);
