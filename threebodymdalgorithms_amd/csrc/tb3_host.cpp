@@ -813,12 +813,12 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
         }
         if (!uses_cutoff(c) && d.n >= 3 && c->precision != TB3_FP64_EXACT) ensure_triples(c, d.n);
         capture_step_graph(c, dt, g);
-        launch_begin_steps(s, c->d_status, c->d_counts);
+        launch_begin_steps(s, c->d_status, c->d_counts,
+                           c->graph_fused && c->graph_slots > 0 ? &c->grid : nullptr);
         if (c->graph_fused) {
             // the first step's predictor and binning; each step_exec_mid
             // replay ends with the next step's
             if (c->graph_slots > 0) {
-                launch_slots_begin(s, c->grid, c->d_status);
                 launch_bin_slotted(s, d, dt, dt * dt * 0.5, 1, c->grid, c->d_status);
             } else {
                 launch_bin_ranked(s, d, 1, dt, dt * dt * 0.5, c->precision == TB3_FP64_EXACT ? 0 : 1, c->grid,

@@ -118,7 +118,7 @@ struct Status {
     double *rep_energy;
     // a cell of the slotted layout received more particles than it has
     // slots (the sweep then fails the pass); cleared only by
-    // launch_slots_begin, never by a status clear, since the binning that
+    // launch_begin_steps for a slotted run, never by a status clear, since the binning that
     // sets it runs beside the clear of the replayed step's last kernel
     int pad_over;
 };
@@ -130,7 +130,9 @@ void launch_rows_from_soa(hipStream_t s, int n, const double *x, const double *y
 void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite);
 // Starts a run of replayed step graphs: clears the status block, halt = 0,
 // nstep = 0, counts = counts.
-void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts);
+// With `slotted` (a run on the slotted layout) it also clears the slot
+// counts, the epoch and Status::pad_over.
+void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts, CellGrid *slotted = nullptr);
 // Folds the slot partial sums into st->count / st->energy and clears the
 // slots; sets st->halt when the sweep overflowed (st->flags[2]).
 void launch_finalize(hipStream_t s, Status *st);
@@ -170,13 +172,11 @@ void launch_scatter_fused(hipStream_t s, const DeviceState &d, CellGrid &g, Stat
 // predictor, force clear (zero_f) and binning.
 void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const double g3[3], Status *st,
                        int sum_roles, double dt, double h, int zero_f, CellGrid &g, int slotted = 0);
-// The slotted layout (CellGrid::pad_cap): launch_slots_begin clears both
-// count arrays, the epoch, the ticket and Status::pad_over before a run of
-// replayed steps; launch_bin_slotted is that run's first predictor (+ force
-// clear) and binning into the slots, with the cell order of the first sweep.
+// The slotted layout (CellGrid::pad_cap): launch_bin_slotted is a run's
+// first predictor (+ force clear) and binning into the slots, with the cell
+// order of the first sweep (after launch_begin_steps with the grid).
 // launch_update_bin and launch_update_fin with `slotted` read and fill the
 // slotted counts instead (and order the next sweep's cells themselves).
-void launch_slots_begin(hipStream_t s, CellGrid &g, Status *st);
 void launch_bin_slotted(hipStream_t s, const DeviceState &d, double dt, double h, int zero_f, CellGrid &g,
                         Status *st);
 

@@ -681,17 +681,22 @@ __global__ void k_bin(DeviceState d, int predict, double dt, double h, int zero_
     else atomicAdd(&cell_cnt[c], 1);
 }
 
-// A run of replayed steps on the slotted layout starts with every count at
-// zero, epoch 0 and a clear overflow flag (a kernel, so that nothing here is
-// a memset).
-__global__ void k_slots_begin(int ncell, SlotsDev sl, Status *st)
+// k_begin_steps for a run on the slotted layout, which also starts with
+// every slot count at zero, epoch 0 and a clear overflow flag (one workgroup;
+// a kernel, so that nothing here is a memset).
+__global__ __launch_bounds__(256) void k_begin_steps_slotted(Status *st, unsigned long long *counts, int ncell,
+                                                             SlotsDev sl)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int i = t; i < 2 * ncell; i += gridDim.x * blockDim.x) sl.cnt[i] = 0;
+    const int t = threadIdx.x;
+    clear_status(st, false, t, blockDim.x);
+    for (int i = t; i < 2 * ncell; i += blockDim.x) sl.cnt[i] = 0;
     if (t == 0) {
+        st->halt = 0;
+        st->nstep = 0;
+        st->counts = counts;
+        st->pad_over = 0;
         sl.state[0] = 0u;
         sl.state[1] = 0u;
-        st->pad_over = 0;
     }
 }
 
@@ -2779,8 +2784,13 @@ void launch_reset_status(hipStream_t s, Status *st, int keep_nonfinite)
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, s, st, keep_nonfinite);
 }
 
-void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts)
+void launch_begin_steps(hipStream_t s, Status *st, unsigned long long *counts, CellGrid *slotted)
 {
+    if (slotted) {
+        hipLaunchKernelGGL(k_begin_steps_slotted, dim3(1), dim3(256), 0, s, st, counts, slotted->ncell,
+                           slots_dev(slotted, 1));
+        return;
+    }
     hipLaunchKernelGGL(k_begin_steps, dim3(1), dim3(64), 0, s, st, counts);
 }
 
@@ -2862,12 +2872,6 @@ void launch_update_bin(hipStream_t s, const DeviceState &d, double hdt, const do
     hipLaunchKernelGGL(k_update_bin, dim3(nb), dim3(NSLOT), 0, s, d, hdt, g3[0], g3[1], g3[2], st, sum_roles, dt,
                        h, zero_f, grid_dev(g), g.cell_of, g.cell_cnt, g.cell_rank, g.ncell, g.cell_start,
                        g.cell_work, g.order, slots_dev(&g, slotted));
-}
-
-void launch_slots_begin(hipStream_t s, CellGrid &g, Status *st)
-{
-    hipLaunchKernelGGL(k_slots_begin, dim3(blocks_for(2LL * g.ncell, 256)), dim3(256), 0, s, g.ncell,
-                       slots_dev(&g, 1), st);
 }
 
 void launch_bin_slotted(hipStream_t s, const DeviceState &d, double dt, double h, int zero_f, CellGrid &g,
