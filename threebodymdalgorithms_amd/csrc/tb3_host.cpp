@@ -789,7 +789,8 @@ void run_steps_graph(tb3_ctx *c, int steps, double dt, const double *g, uint64_t
     if ((long long)steps * per > c->counts_cap) {
         // room for 4096 counts at least, and twice what is asked: a run
         // longer than the last one must not pay for a device and a pinned
-        // host allocation (~0.15 ms together) on its own clock
+        // host allocation on its own clock (they cost a 20-step run about
+        // 4 us per step)
         const long long cap = std::max<long long>(2LL * steps * per, 4096);
         dev_alloc(c->d_counts, cap);
         if (c->h_counts) (void)hipHostFree(c->h_counts);
