@@ -101,6 +101,18 @@ def test_run_equals_steps_p3bca(eng, precision):
             assert_same(got3, cgot3, ref2, cref2, False)
 
 
+def test_run_uses_every_batched_graph(eng):
+    """A run of 32 steps replays its 30 middle steps as batched graphs of 16,
+    8, 4 and 2 steps: on the exact path the run equals 32 single steps bit
+    for bit, so every batched graph replays the same step."""
+    P = uniform_particles(1500, (1500 / 2.0) ** (1 / 3), 46)
+    V = np.zeros_like(P)
+    say("run of 32 steps vs 32 steps, exact")
+    ref, cref = state_after(eng, P, V, "p3bca", 2.0, "exact", [("step", 1)] * 32)
+    got, cgot = state_after(eng, P, V, "p3bca", 2.0, "exact", [("run", 32)])
+    assert_same(got, cgot, ref, cref, True)
+
+
 def test_run_equals_steps_auta(eng):
     P = uniform_particles(300, 5.0, 43)
     V = np.zeros_like(P)
